@@ -1,0 +1,353 @@
+"""Pins for the CPU oracle (oracle/vpfp_oracle.py).
+
+Each check is chosen so that a plausible mistake in the restatement (a sign,
+an index, a dropped term, a wrong axis or normalisation) fails: closed forms,
+brute-force sums on tiny grids, exact discrete identities, the reference's own
+known-answer values and golden vectors written from the reference itself.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import vpfp_oracle as O
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------- advection
+
+
+def _grid(nx, nv, k, vmax):
+    return O.Grid.make(nx, nv, 2 * np.pi / k, vmax)
+
+
+def test_vdfdx_is_exact_translation_of_bandlimited_data():
+    """f = sum_m A_m cos(m k x + phi_m) g(v) -> same with x -> x - v dt."""
+    g = _grid(32, 24, 0.3, 6.0)
+    rng = np.random.default_rng(1)
+    modes = [(m, rng.normal(), rng.uniform(0, 2 * np.pi)) for m in range(0, 15)]
+    gv = np.exp(-g.v ** 2 / 3.0) * (1 + 0.3 * g.v)
+
+    def prof(shift):
+        xs = g.x[:, None] - shift
+        return sum(a * np.cos(m * 0.3 * xs + p) for m, a, p in modes) * gv[None, :]
+
+    dt = 0.37
+    got = O.vdfdx(prof(0.0), dt, g.kx, g.v)
+    want = prof(g.v[None, :] * dt)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+def test_edfdv_is_exact_translation_of_periodic_data():
+    """f periodic band-limited in v -> f(x, v - e(x) dt)."""
+    nv, vmax = 32, 4.0
+    g = _grid(8, nv, 0.5, vmax)
+    kappa = 2 * np.pi / (2 * vmax)
+    rng = np.random.default_rng(2)
+    modes = [(m, rng.normal(), rng.uniform(0, 2 * np.pi)) for m in range(0, 15)]
+    e = rng.normal(size=8)
+
+    def prof(vv):
+        return sum(a * np.cos(m * kappa * vv + p) for m, a, p in modes)
+
+    dt = 0.21
+    f = np.repeat(prof(g.v)[None, :], 8, axis=0)
+    got = O.edfdv(f, e, dt, g.kv)
+    want = prof(g.v[None, :] - e[:, None] * dt)
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+
+
+def test_nyquist_bin_is_scaled_by_cosine():
+    """Taking the real part leaves the Nyquist mode scaled by cos(k_N c)."""
+    g = _grid(16, 8, 0.4, 3.0)
+    alt = (-1.0) ** np.arange(16)
+    f = alt[:, None] * np.ones(8)[None, :]
+    dt = 0.3
+    got = O.vdfdx(f, dt, g.kx, g.v)
+    want = alt[:, None] * np.cos(g.kx[8] * dt * g.v)[None, :]
+    np.testing.assert_allclose(got, want, atol=1e-13)
+
+    f = np.ones(16)[:, None] * ((-1.0) ** np.arange(8))[None, :]
+    e = np.linspace(-1, 1, 16)
+    got = O.edfdv(f, e, dt, g.kv)
+    want = np.cos(g.kv[4] * dt * e)[:, None] * ((-1.0) ** np.arange(8))[None, :]
+    np.testing.assert_allclose(got, want, atol=1e-13)
+
+
+def test_pushers_match_bruteforce_dft_sums():
+    """Explicit O(N^2) DFT sums with the fftfreq wavenumbers on a tiny grid."""
+    nx, nv = 6, 10
+    g = _grid(nx, nv, 0.7, 2.5)
+    rng = np.random.default_rng(3)
+    f = rng.normal(size=(nx, nv))
+    e = rng.normal(size=nx)
+    dt = 0.45
+    jx = np.arange(nx)
+    jv = np.arange(nv)
+    # x-advection: f'(i, v) = Re (1/nx) sum_m sum_l f(l, v) e^{-2pi i m l/nx} e^{-i kx_m v dt} e^{2pi i m i/nx}
+    want = np.zeros((nx, nv))
+    for i in range(nx):
+        for c in range(nv):
+            acc = 0j
+            for m in range(nx):
+                fm = sum(f[l, c] * np.exp(-2j * np.pi * m * l / nx) for l in jx)
+                acc += fm * np.exp(-1j * g.kx[m] * dt * g.v[c]) * np.exp(2j * np.pi * m * i / nx)
+            want[i, c] = (acc / nx).real
+    np.testing.assert_allclose(O.vdfdx(f, dt, g.kx, g.v), want, atol=1e-13)
+    want = np.zeros((nx, nv))
+    for r in range(nx):
+        for i in range(nv):
+            acc = 0j
+            for m in range(nv):
+                fm = sum(f[r, l] * np.exp(-2j * np.pi * m * l / nv) for l in jv)
+                acc += fm * np.exp(-1j * g.kv[m] * dt * e[r]) * np.exp(2j * np.pi * m * i / nv)
+            want[r, i] = (acc / nv).real
+    np.testing.assert_allclose(O.edfdv(f, e, dt, g.kv), want, atol=1e-13)
+
+
+# ---------------------------------------------------------------- field
+
+
+def test_field_closed_forms_of_reference_test():
+    """tests/test_fieldsolver.py:27-54 of the reference (nx=96, k=0.25)."""
+    nx, kp = 96, 0.25
+    _, x, kx, okx = O.spatial_grid(0.0, 2 * np.pi / kp, nx)
+    dens = [1 + np.sin(kp * x), 1 + np.cos(2 * kp * x), 1 + np.sin(2 * kp * x) + np.cos(8 * kp * x)]
+    fields = [np.cos(kp * x) / kp, -np.sin(2 * kp * x) / 2 / kp,
+              np.cos(2 * kp * x) / 2 / kp - np.sin(8 * kp * x) / 8 / kp]
+    for n, want in zip(dens, fields):
+        # the reference test passes 1 - rho, so the solver sees rho - 1 + 1
+        got = O.solve_for_field(1.0 - n, okx)
+        np.testing.assert_allclose(got, want, atol=1e-12)
+
+
+def test_trapz_closed_form():
+    dv, v, _ = O.velocity_grid(3.0, 40)
+    g = 2.0 + 0.5 * v  # linear: trapz exact
+    want = 2.0 * (v[-1] - v[0]) + 0.25 * (v[-1] ** 2 - v[0] ** 2)
+    assert abs(O.trapz_v(g, dv) - want) < 1e-13
+    f = np.ones((3, 40))
+    np.testing.assert_allclose(O.compute_charges(f, dv), (v[-1] - v[0]) * np.ones(3), rtol=1e-14)
+
+
+def test_field_solve_adds_driver():
+    g = _grid(16, 8, 0.3, 4.0)
+    f = np.ones((16, 8)) / (8 * g.dv)
+    drv = np.sin(0.3 * g.x)
+    np.testing.assert_allclose(O.field_solve(drv, f, g.dv, g.one_over_kx),
+                               drv + O.solve_for_field(O.compute_charges(f, g.dv), g.one_over_kx))
+
+
+# ---------------------------------------------------------------- collisions
+
+
+def _coll_init(vshift, nx=2, nv=1024, vmax=6.0):
+    dv, v, _ = O.velocity_grid(vmax, nv)
+    f = np.repeat(np.exp(-((v - vshift) ** 2.0) / 2.0)[None, :], nx, axis=0)
+    f = f / O.trapz_v(f, dv)[:, None]
+    return f, v, dv
+
+
+def test_thomas_matches_dense_solve():
+    rng = np.random.default_rng(5)
+    nx, n = 4, 50
+    a = rng.normal(size=(nx, n - 1))
+    c = rng.normal(size=(nx, n - 1))
+    b = 4.0 + np.abs(rng.normal(size=(nx, n)))
+    d = rng.normal(size=(nx, n))
+    np.testing.assert_allclose(O.thomas(a, b, c, d), O.dense_tridiag_solve(a, b, c, d), rtol=1e-12, atol=1e-13)
+
+
+@pytest.mark.parametrize("op", ["lb", "dg"])
+def test_reference_collision_invariants(op):
+    """tests/test_collisions.py:39-165 of the reference (nu=1e-2, dt=0.1, 16 steps)."""
+    nu, dt = 1e-2, 0.1
+
+    def run(vshift):
+        f, v, dv = _coll_init(vshift)
+        fo = f.copy()
+        for _ in range(16):
+            fo = O.collide(fo, v, nu, dt, dv, op)
+        return f, fo, v, dv
+
+    f, fo, v, dv = run(0.0)
+    np.testing.assert_almost_equal(f, fo, decimal=4)
+    np.testing.assert_almost_equal(O.trapz_v(f * v, dv), O.trapz_v(fo * v, dv), decimal=4)
+    f, fo, v, dv = run(0.5)
+    np.testing.assert_almost_equal(O.trapz_v(fo * v ** 2, dv), O.trapz_v(f * v ** 2, dv), decimal=4)
+    np.testing.assert_almost_equal(O.trapz_v(fo, dv), O.trapz_v(f, dv), decimal=4)
+    f, fo, v, dv = run(1.5)
+    p_in, p_out = O.trapz_v(f * v, dv), O.trapz_v(fo * v, dv)
+    if op == "lb":
+        assert np.all(p_out < p_in)
+    else:
+        np.testing.assert_almost_equal(p_in, p_out, decimal=4)
+
+
+@pytest.mark.parametrize("op", ["lb", "dg"])
+def test_collision_exact_discrete_identities(op):
+    """Column sums of the implicit matrix give exact moment identities.
+
+    For interior columns j of A = I - nu dt L:  sum_i A_ij = 1,
+    sum_i v_i A_ij = v_j + nu dt (v_j - vbar),
+    sum_i v_i^2 A_ij = v_j^2 - 2 nu dt T + 2 nu dt (v_j - vbar) v_j,
+    with vbar and T the un-normalised moments of the input f.  f vanishes in
+    100 cells at each edge so the boundary columns do not contribute.  The
+    input density is 1.7, so a temperature divided by n fails the energy line.
+    """
+    nv, vmax, nu, dt = 1024, 6.0, 1e-3, 0.1
+    dv, v, _ = O.velocity_grid(vmax, nv)
+    f = 1.7 * np.exp(-((v - 0.6) ** 2) / 2.0 / 1.2)[None, :] * np.array([1.0, 0.7])[:, None]
+    f[:, :100] = 0.0
+    f[:, -100:] = 0.0
+    fo = O.collide(f, v, nu, dt, dv, op)
+    if op == "lb":
+        vbar = np.zeros(2)
+        temp = O.trapz_v(f * v ** 2, dv)
+    else:
+        vbar = O.trapz_v(f * v, dv)
+        temp = O.trapz_v(f * (v - vbar[:, None]) ** 2, dv)
+    nd = nu * dt
+    s0 = fo.sum(1)
+    s1 = (fo * (v + nd * (v - vbar[:, None]))).sum(1)
+    s2 = (fo * (v ** 2 - 2 * nd * temp[:, None] + 2 * nd * (v - vbar[:, None]) * v)).sum(1)
+    np.testing.assert_allclose(f.sum(1), s0, rtol=1e-13)
+    np.testing.assert_allclose((f * v).sum(1), s1, rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose((f * v ** 2).sum(1), s2, rtol=1e-12)
+
+
+def test_collision_notebook_known_answers():
+    """notebooks/test_fokker_planck.ipynb cell 5: LB, nu=1e-2, dt=0.1, 99 steps on
+    f (1 + 0.05 cos(64 pi v / 12)); printed dn=8.55e-09, dp=-1.11e-16, dE=3.09e-07."""
+    f, v, dv = _coll_init(0.0)
+    f = f * (1 + 0.05 * np.cos(64 * np.pi / 12 * v))
+    fo = f.copy()
+    for _ in range(99):
+        fo = O.collide(fo, v, 1e-2, 0.1, dv, "lb")
+    dn = np.mean(O.trapz_v(f, dv) - O.trapz_v(fo, dv))
+    dp = np.mean(O.trapz_v(f * v, dv) - O.trapz_v(fo * v, dv))
+    de = np.mean(O.trapz_v(f * v ** 2, dv) - O.trapz_v(fo * v ** 2, dv))
+    assert float(np.format_float_scientific(dn, precision=2)) == pytest.approx(8.55e-9, rel=1e-12)
+    assert float(np.format_float_scientific(de, precision=2)) == pytest.approx(3.09e-7, rel=1e-12)
+    assert abs(dp) < 1e-14
+
+
+def test_collide_nu_rules():
+    f, v, dv = _coll_init(0.0, nx=1, nv=16)
+    assert O.collide(f, v, 0.0, 0.1, dv) is f
+    with pytest.raises(NotImplementedError):
+        O.collide(f, v, -1.0, 0.1, dv)
+
+
+# ---------------------------------------------------------------- integrators
+
+
+def _landau_state(nx, nv, k, amp, vmax=6.4):
+    g = _grid(nx, nv, k, vmax)
+    f = O.perturbed_maxwellian(g.x, g.v, k, amp)
+    return g, f
+
+
+@pytest.mark.parametrize("name,order,dts", [("leapfrog", 2, (0.4, 0.2)),
+                                            ("pefrl", 4, (0.8, 0.4)),
+                                            ("h-sixth", 6, (1.0, 0.5))])
+def test_integrator_order(name, order, dts):
+    g, f0 = _landau_state(16, 128, 0.35, 0.05)
+    t_end = 4.0
+
+    def run(dt):
+        # a consistent incoming field: E of f0 (the reference's e = 0 start is
+        # a first-order error of its own)
+        e, f = O.field_solve(0.0, f0, g.dv, g.one_over_kx), f0.copy()
+        for i in range(int(round(t_end / dt))):
+            e, f = O.INTEGRATORS[name](g, e, f, i * dt, dt)
+        return e
+
+    ref = run(dts[1] / 8)
+    e1, e2 = run(dts[0]), run(dts[1])
+    r = np.abs(e1 - ref).max() / np.abs(e2 - ref).max()
+    assert np.log2(r) > order - 0.6, (name, np.log2(r))
+
+
+def test_landau_damping_rate_matches_canosa():
+    """64x512 collisionless Landau run at k=0.3: gamma and omega of <E^2> peaks."""
+    g, f = _landau_state(64, 512, 0.3, 1e-4)
+    dt, n = 0.1, 600
+    e, f, eh = O.run_steps(g, np.zeros(64), f, n, dt)
+    t = dt * (np.arange(n) + 1)
+    gamma, omega = O.damping_rate_and_frequency(np.mean(eh ** 2, axis=1), t, t_min=8.0)
+    assert gamma == pytest.approx(-0.012623, rel=0.02)
+    assert omega == pytest.approx(1.1598, rel=0.005)
+
+
+def test_z_function_roots_canosa_table():
+    tab = json.load(open(os.path.join(GOLD, "canosa_1973.json")))
+    for k, wr, wi in zip(tab["k0"], tab["w_real"], tab["w_imag"]):
+        w = O.epw_root(k)
+        np.testing.assert_almost_equal(w, wr + 1j * wi, decimal=4)
+
+
+# ---------------------------------------------------------------- golden vectors
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return dict(np.load(os.path.join(GOLD, "vlapy_core_small.npz")))
+
+
+def _gold_grid(z):
+    return O.Grid(z["x"], z["kx"], z["one_over_kx"], z["v"], z["kv"], float(z["dv"]))
+
+
+def _gold_driver(z):
+    x = z["x"]
+    pulses = {"p": {"k0": 0.35, "w0": 1.2, "t_L": 0.0, "t_R": 20.0, "t_wL": 2.5, "t_wR": 2.5, "a0": 0.05}}
+    return lambda t: O.driver_field(x, pulses, t)
+
+
+def test_golden_operators(gold):
+    z = gold
+    g = _gold_grid(z)
+    dt, t0 = float(z["dt"]), float(z["t0"])
+    np.testing.assert_allclose(O.vdfdx(z["f0"], dt, g.kx, g.v), z["vdfdx_f"], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(O.edfdv(z["f0"], z["e0"], dt, g.kv), z["edfdv_f"], rtol=0, atol=1e-14)
+    drv = _gold_driver(z)
+    np.testing.assert_allclose(O.field_solve(drv(t0), z["f0"], g.dv, g.one_over_kx), z["field_e"], atol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["leapfrog", "pefrl", "h-sixth"])
+def test_golden_integrators(gold, name):
+    z = gold
+    g = _gold_grid(z)
+    e, f = O.INTEGRATORS[name](g, z["e0"], z["f0"], float(z["t0"]), float(z["dt"]), _gold_driver(z))
+    np.testing.assert_allclose(e, z[name + "_e"], rtol=0, atol=1e-13)
+    np.testing.assert_allclose(f, z[name + "_f"], rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("op", ["lb", "dg"])
+def test_golden_collision(gold, op):
+    z = gold
+    fo = O.collide(z["coll_" + op + "_in"], z["v"], 3e-2, 0.2, float(z["dv"]), op)
+    np.testing.assert_allclose(fo, z["coll_" + op + "_out"], rtol=1e-13, atol=1e-15)
+
+
+def test_golden_timestep_and_storage(gold):
+    z = gold
+    g = _gold_grid(z)
+    nt, dt, t0 = 3, float(z["dt"]), float(z["t0"])
+    temp = O.new_temp_storage(z["e0"], z["f0"], nt)
+    tb = t0 + dt * np.arange(nt)
+    temp = O.inner_loop(g, temp, tb, z["ts_driver"], dt, 1e-2, "dg", "leapfrog", _gold_driver(z))
+    np.testing.assert_allclose(temp["f"], z["ts_f"], atol=1e-13)
+    np.testing.assert_allclose(temp["e"], z["ts_e"], atol=1e-13)
+    # high moments cancel: compare against the integrand scale sum |f v^p| dv
+    scale = {k: O.trapz_v(np.abs(z["ts_f"]) * np.abs(g.v) ** p, g.dv).max()
+             for k, p in (("n", 0), ("j", 1), ("T", 2), ("q", 3), ("fv4", 4), ("vN", 5))}
+    for k in O.FIELD_KEYS:
+        atol = 1e-14 * scale.get(k, 1.0) + 1e-14
+        np.testing.assert_allclose(temp["fields"][k], z["ts_fields_" + k], rtol=1e-12, atol=atol, err_msg=k)
+    for k in O.SERIES_KEYS:
+        np.testing.assert_allclose(temp["series"][k], z["ts_series_" + k], rtol=1e-12, atol=1e-13, err_msg=k)
