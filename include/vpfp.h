@@ -1,0 +1,123 @@
+/*
+ * vpfp.h - C ABI of the MI355X (gfx950) Vlasov-Poisson-Fokker-Planck step.
+ *
+ * One library, libvpfp.so, built with hipcc for gfx950.  Every entry point
+ * takes plain device pointers and sizes, enqueues HIP kernels on the given
+ * stream (a hipStream_t passed as void*, NULL = default stream) and returns
+ * an int status; nothing is synchronised unless stated.  The reference
+ * (VlaPy, pure NumPy) binds these through ctypes; the Python bindings live in
+ * vlapy_amd/_lib.py.
+ *
+ * Array conventions follow vlapy.core: f is (nx, nv) row-major with v
+ * fastest, stored as double (VPFP_F64) or float (VPFP_F32); fields, moments,
+ * grids and collision diagonals are always double.
+ *
+ * Status codes map to the reference's Python exceptions:
+ *   VPFP_OK          0
+ *   VPFP_EBADARG     1  -> ValueError
+ *   VPFP_EUNSUPPORTED 2 -> NotImplementedError (name or size the GPU path
+ *                          does not implement)
+ *   VPFP_EHIP        3  -> RuntimeError (HIP runtime error)
+ * vpfp_last_error() returns a message for the last failure on this thread.
+ */
+#ifndef VPFP_H
+#define VPFP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VPFP_OK 0
+#define VPFP_EBADARG 1
+#define VPFP_EUNSUPPORTED 2
+#define VPFP_EHIP 3
+
+#define VPFP_F64 0
+#define VPFP_F32 1
+
+#define VPFP_OP_LB 0 /* Lenard-Bernstein, collisions.py:26-83 */
+#define VPFP_OP_DG 1 /* Dougherty, collisions.py:86-160 */
+
+#define VPFP_NMOMENTS 8 /* n, j, T, q, fv4, vN, f^2, f log f (step.py:164-171,219-224) */
+
+typedef struct vpfp_plan_s* vpfp_plan_t;
+
+/* Grid and operator description; host arrays, copied at plan creation.
+ * Replaces the captured `stuff_for_time_loop` of the reference factories
+ * (outer_loop.py:125-144): kx, one_over_kx (nx), v, kv (nv), dv, nu, dt and
+ * the collision operator name all_params["fokker-planck"]["type"]. */
+typedef struct {
+  int nx;
+  int nv;
+  int precision;              /* VPFP_F64 or VPFP_F32 (f storage and advection) */
+  const double* kx;           /* [nx] fftfreq(nx, dx) * 2 pi */
+  const double* one_over_kx;  /* [nx] 1/kx, 0 at DC */
+  const double* v;            /* [nv] cell-centred velocities */
+  const double* kv;           /* [nv] fftfreq(nv, dv) * 2 pi */
+  double dv;
+  double nu;                  /* collision frequency (>= 0) */
+  double dt;                  /* time step used by vpfp_collide / vpfp_step_n */
+  int op;                     /* VPFP_OP_LB or VPFP_OP_DG */
+} vpfp_grid_desc;
+
+/* Plan creation: copies the grid to the device, builds twiddle tables and
+ * scratch.  Replaces the factory calls get_vdfdx / get_edfdv /
+ * get_field_solver / get_collision_step (vlasov.py:213-260, field.py:91-107,
+ * step.py:70-113).  Checks that kx and kv are fftfreq tables. */
+int vpfp_plan_create(const vpfp_grid_desc* desc, vpfp_plan_t* out);
+int vpfp_plan_destroy(vpfp_plan_t plan);
+
+/* x-advection f_out = Re ifft_x(exp(-i kx v dt) fft_x f_in).
+ * Replaces step_vdfdx_exponential, vlasov.py:94-108.  f_out may equal f_in. */
+int vpfp_vdfdx(vpfp_plan_t plan, const void* f_in, void* f_out, double dt, void* stream);
+
+/* v-advection f_out = Re ifft_v(exp(-i kv e dt) fft_v f_in).
+ * Replaces step_edfdv_exponential, vlasov.py:123-138.  e: [nx] device. */
+int vpfp_edfdv(vpfp_plan_t plan, const void* f_in, void* f_out, const double* e, double dt, void* stream);
+
+/* n(x) = trapz_v f.  Replaces field.compute_charges, field.py:27-36. */
+int vpfp_compute_charges(vpfp_plan_t plan, const void* f, double* n_out, void* stream);
+
+/* E = Re ifft(i one_over_kx fft(1 - n)).  Replaces field.solve_for_field,
+ * field.py:52-63.  Any nx >= 2 (direct DFT when nx is not a power of two). */
+int vpfp_solve_for_field(vpfp_plan_t plan, const double* charge_density, double* e_out, void* stream);
+
+/* E = driver + solve_for_field(compute_charges(f)).  Replaces
+ * solve_total_electric_field, field.py:77-86.  driver may be NULL (zero). */
+int vpfp_field(vpfp_plan_t plan, const void* f, const double* driver, double* e_out, void* stream);
+
+/* Collision diagonals a [nx][nv-1], b [nx][nv], c [nx][nv-1] of f.
+ * Replaces make_philharmonic_arrays_for_matrix / make_dougherty_arrays_for_matrix,
+ * collisions.py:44-81, 104-158 (plan op, nu, dt, dv). */
+int vpfp_collision_diagonals(vpfp_plan_t plan, const void* f, double* a, double* b, double* c, void* stream);
+
+/* Batched tridiagonal solve x = A^-1 d for every row, A = diag(a,-1) +
+ * diag(b) + diag(c,1).  Replaces _batched_tridiag_solver_, collisions.py:232-263
+ * (and the per-x dense solve of collisions.py:172-217). x may equal d. */
+int vpfp_tridiag_solve(vpfp_plan_t plan, const double* a, const double* b, const double* c, const void* d,
+                       void* x, void* stream);
+
+/* One implicit collision step with the plan's nu, dt and operator.
+ * Replaces take_collision_step, step.py:80-108 (identity when nu == 0). */
+int vpfp_collide(vpfp_plan_t plan, const void* f_in, void* f_out, void* stream);
+
+/* n_steps leapfrog Vlasov-Poisson steps each followed by the collision
+ * step, the fused inner loop of outer_loop.py:248-274 with step.py:302-326:
+ *   kick(e, dt/2); drift(dt); E = driver_i + field(f); kick(E, dt/2);
+ *   collide; moments.
+ * f [nx][nv] and e [nx] are updated in place (e becomes the last E).
+ * driver [n_steps][nx] (field added at step i) may be NULL.
+ * e_hist [n_steps][nx], moments [n_steps][VPFP_NMOMENTS][nx] (trapz moments
+ * of f after the step) and series [n_steps][2] (mean E^2, mean driver^2) are
+ * optional outputs (NULL to skip). */
+int vpfp_step_n(vpfp_plan_t plan, void* f, double* e, int n_steps, const double* driver, double* e_hist,
+                double* moments, double* series, void* stream);
+
+const char* vpfp_last_error(void);
+int vpfp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VPFP_H */

@@ -1,0 +1,209 @@
+"""GPU parity of each C-ABI entry point against the CPU oracle.
+
+FP64 bar: max |gpu - oracle| / max |oracle| <= 1e-12 (per call).
+FP32 storage/advection bar (moments and tridiagonal in FP64): 2e-6 per call.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import vpfp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12
+F32_TOL = 2e-6
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+
+    assert torch.cuda.is_available(), "no GPU visible"
+    from vlapy_amd import _lib
+
+    _lib.library()
+    return torch.device("cuda:0")
+
+
+def T(a, dev, dtype=None):
+    import torch
+
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype or torch.float64, device=dev)
+
+
+def make_plan(g, nx, nv, precision="fp64", nu=0.0, dt=0.0, op="lb"):
+    from vlapy_amd._lib import Plan
+
+    return Plan(nx, nv, precision, g.kx, g.one_over_kx, g.v, g.kv, g.dv, nu, dt, op)
+
+
+SIZES = [(4, 8), (8, 16), (16, 64), (64, 512), (64, 1024), (32, 2048), (8, 8192), (1024, 64), (3, 256)]
+
+
+def _fdtype(prec):
+    import torch
+
+    return torch.float64 if prec == "fp64" else torch.float32
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("nx,nv", SIZES)
+def test_edfdv(dev, nx, nv, prec):
+    from vlapy_amd import ops
+
+    print("edfdv", nx, nv, prec, flush=True)
+    rng = np.random.default_rng(nx * 7 + nv)
+    g = O.Grid.make(nx, nv, 2 * np.pi / 0.3, 6.4)
+    f = 1 + 0.3 * rng.standard_normal((nx, nv))
+    e = rng.standard_normal(nx)
+    p = make_plan(g, nx, nv, prec)
+    fi = T(f, dev, _fdtype(prec))
+    out = ops.edfdv(p, fi, fi.clone(), T(e, dev), 0.17)
+    want = O.edfdv(fi.double().cpu().numpy(), e, 0.17, g.kv)
+    assert rel(out.double().cpu().numpy(), want) < (F64_TOL if prec == "fp64" else F32_TOL)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("nx,nv", [s for s in SIZES if s[0] >= 4])
+def test_vdfdx(dev, nx, nv, prec):
+    from vlapy_amd import ops
+
+    print("vdfdx", nx, nv, prec, flush=True)
+    rng = np.random.default_rng(nx * 5 + nv)
+    g = O.Grid.make(nx, nv, 2 * np.pi / 0.3, 6.4)
+    f = 1 + 0.3 * rng.standard_normal((nx, nv))
+    p = make_plan(g, nx, nv, prec)
+    fi = T(f, dev, _fdtype(prec))
+    out = ops.vdfdx(p, fi, fi.clone(), 0.37)
+    want = O.vdfdx(fi.double().cpu().numpy(), 0.37, g.kx, g.v)
+    assert rel(out.double().cpu().numpy(), want) < (F64_TOL if prec == "fp64" else F32_TOL)
+
+
+@pytest.mark.parametrize("nx,nv", [(4, 8), (64, 512), (96, 32), (7, 16), (1024, 64)])
+def test_field_and_charges(dev, nx, nv):
+    import torch
+    from vlapy_amd import ops
+
+    print("field", nx, nv, flush=True)
+    rng = np.random.default_rng(nx + nv)
+    g = O.Grid.make(nx, nv, 2 * np.pi / 0.25, 5.0)
+    f = np.abs(1 + 0.2 * rng.standard_normal((nx, nv))) / (nv * g.dv)
+    drv = 0.01 * rng.standard_normal(nx)
+    p = make_plan(g, nx, nv)
+    n = ops.compute_charges(p, T(f, dev), torch.zeros(nx, dtype=torch.float64, device=dev))
+    assert rel(n.cpu().numpy(), O.compute_charges(f, g.dv)) < F64_TOL
+    e = ops.field(p, T(f, dev), T(drv, dev), torch.zeros(nx, dtype=torch.float64, device=dev))
+    assert rel(e.cpu().numpy(), O.field_solve(drv, f, g.dv, g.one_over_kx)) < 1e-11
+    rho = 1 + 0.5 * np.sin(0.25 * g.x) + 0.2 * np.cos(2 * 0.25 * g.x)
+    e2 = ops.solve_for_field(p, T(rho, dev), torch.zeros(nx, dtype=torch.float64, device=dev))
+    assert rel(e2.cpu().numpy(), O.solve_for_field(rho, g.one_over_kx)) < F64_TOL
+
+
+def test_field_closed_forms_reference(dev):
+    """tests/test_fieldsolver.py:27-54 of the reference on the GPU (nx=96)."""
+    import torch
+    from vlapy_amd import ops
+
+    nx, kp = 96, 0.25
+    _, x, kx, okx = O.spatial_grid(0.0, 2 * np.pi / kp, nx)
+    from vlapy_amd._lib import Plan
+
+    p = Plan(nx, 8, "fp64", kx, okx, np.zeros(8), np.fft.fftfreq(8) * 2 * np.pi, 1.0)
+    dens = [1 + np.sin(kp * x), 1 + np.cos(2 * kp * x), 1 + np.sin(2 * kp * x) + np.cos(8 * kp * x)]
+    want = [np.cos(kp * x) / kp, -np.sin(2 * kp * x) / 2 / kp, np.cos(2 * kp * x) / 2 / kp - np.sin(8 * kp * x) / 8 / kp]
+    for n, w in zip(dens, want):
+        e = ops.solve_for_field(p, T(1.0 - n, dev), torch.zeros(nx, dtype=torch.float64, device=dev))
+        np.testing.assert_allclose(e.cpu().numpy(), w, atol=1e-12)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("op", ["lb", "dg"])
+@pytest.mark.parametrize("nx,nv", [(2, 1024), (8, 8), (64, 1024), (16, 8192), (5, 256)])
+def test_collide(dev, nx, nv, op, prec):
+    from vlapy_amd import ops
+
+    print("collide", nx, nv, op, prec, flush=True)
+    rng = np.random.default_rng(nx + 3 * nv)
+    g = O.Grid.make(nx, nv, 2 * np.pi / 0.3, 6.0)
+    f = O.perturbed_maxwellian(g.x, g.v, 0.3, 0.2) * (1 + 0.05 * rng.standard_normal((nx, nv)))
+    f = np.abs(f)
+    nu, dt = 1e-2, 0.1
+    p = make_plan(g, nx, nv, prec, nu=nu, dt=dt, op=op)
+    fi = T(f, dev, _fdtype(prec))
+    out = ops.collide(p, fi, fi.clone())
+    want = O.collide(fi.double().cpu().numpy(), g.v, nu, dt, g.dv, op)
+    assert rel(out.double().cpu().numpy(), want) < (F64_TOL if prec == "fp64" else F32_TOL)
+
+
+@pytest.mark.parametrize("op", ["lb", "dg"])
+def test_diagonals_and_tridiag(dev, op):
+    import torch
+    from vlapy_amd import ops
+
+    nx, nv = 6, 512
+    g = O.Grid.make(nx, nv, 2 * np.pi / 0.3, 6.0)
+    rng = np.random.default_rng(11)
+    f = np.abs(O.perturbed_maxwellian(g.x, g.v, 0.3, 0.3) * (1 + 0.1 * rng.standard_normal((nx, nv))))
+    p = make_plan(g, nx, nv, nu=3e-2, dt=0.2, op=op)
+    z = lambda *s: torch.zeros(*s, dtype=torch.float64, device=dev)  # noqa: E731
+    a, b, c = ops.collision_diagonals(p, T(f, dev), z(nx, nv - 1), z(nx, nv), z(nx, nv - 1))
+    wa, wb, wc = O.collision_diagonals(f, g.v, 3e-2, 0.2, g.dv, op)
+    for got, want in ((a, wa), (b, wb), (c, wc)):
+        assert rel(got.cpu().numpy(), want) < F64_TOL
+    x = ops.tridiag_solve(p, a, b, c, T(f, dev), z(nx, nv))
+    assert rel(x.cpu().numpy(), O.thomas(wa, wb, wc, f)) < F64_TOL
+
+
+@pytest.mark.parametrize("nu,op", [(0.0, "lb"), (1e-2, "dg"), (1e-2, "lb")])
+def test_step_n_matches_oracle_loop(dev, nu, op):
+    import torch
+    from vlapy_amd import ops
+
+    nx, nv, k, dt, n = 16, 64, 0.35, 0.1, 12
+    print("step_n", nu, op, flush=True)
+    g = O.Grid.make(nx, nv, 2 * np.pi / k, 6.0)
+    f0 = O.perturbed_maxwellian(g.x, g.v, k, 0.2)
+    rng = np.random.default_rng(3)
+    e0 = 0.01 * rng.standard_normal(nx)
+    drv = 1e-3 * rng.standard_normal((n, nx))
+    p = make_plan(g, nx, nv, nu=nu, dt=dt, op=op)
+    f = T(f0, dev)
+    e = T(e0, dev)
+    eh = torch.zeros(n, nx, dtype=torch.float64, device=dev)
+    mo = torch.zeros(n, 8, nx, dtype=torch.float64, device=dev)
+    se = torch.zeros(n, 2, dtype=torch.float64, device=dev)
+    ops.step_n(p, f, e, n, T(drv, dev), eh, mo, se)
+    # oracle
+    ef, ff = e0.copy(), f0.copy()
+    for i in range(n):
+        ef, ff = O.leapfrog_step(g, ef, ff, i * dt, dt, driver=lambda t, i=i: drv[i])
+        ff = O.collide(ff, g.v, nu, dt, g.dv, op)
+        assert rel(eh[i].cpu().numpy(), ef) < 1e-11, i
+        m = O.step_moments(ff, g.v, g.dv)
+        for q, key in enumerate(("n", "j", "T", "q", "fv4", "vN")):
+            scale = O.trapz_v(np.abs(ff) * np.abs(g.v) ** q, g.dv).max()
+            assert np.abs(mo[i, q].cpu().numpy() - m[key]).max() <= 1e-12 * scale, (i, key)
+        assert abs(se[i, 0].item() - np.mean(ef ** 2)) <= 1e-11 * np.mean(ef ** 2)
+        assert abs(se[i, 1].item() - np.mean(drv[i] ** 2)) <= 1e-12 * np.mean(drv[i] ** 2)
+    assert rel(f.cpu().numpy(), ff) < F64_TOL
+    assert rel(e.cpu().numpy(), ef) < 1e-11
+
+
+def test_unsupported_sizes_raise(dev):
+    import torch
+    from vlapy_amd import ops
+
+    g = O.Grid.make(6, 12, 2 * np.pi / 0.3, 6.0)
+    p = make_plan(g, 6, 12)
+    f = torch.zeros(6, 12, dtype=torch.float64, device=dev)
+    with pytest.raises(NotImplementedError):
+        ops.vdfdx(p, f, f, 0.1)
+    with pytest.raises(NotImplementedError):
+        ops.edfdv(p, f, f, torch.zeros(6, dtype=torch.float64, device=dev), 0.1)
+    with pytest.raises(ValueError):
+        ops.edfdv(p, f, f, torch.zeros(5, dtype=torch.float64, device=dev), 0.1)

@@ -1,0 +1,133 @@
+"""ctypes binding of libvpfp.so (the C ABI declared in include/vpfp.h).
+
+The library is built in-tree (vlapy_amd/libvpfp.so, see vlapy_amd/build.py).
+There is no fallback: if the library is missing or no GPU is visible, every
+entry point raises.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvpfp.so")
+
+VPFP_OK, VPFP_EBADARG, VPFP_EUNSUPPORTED, VPFP_EHIP = 0, 1, 2, 3
+VPFP_F64, VPFP_F32 = 0, 1
+VPFP_OP_LB, VPFP_OP_DG = 0, 1
+NMOMENTS = 8
+MOMENT_NAMES = ("n", "j", "T", "q", "fv4", "vN", "f2", "flogf")
+
+_c_p = ctypes.c_void_p
+_c_d = ctypes.c_double
+_c_i = ctypes.c_int
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [
+        ("nx", _c_i),
+        ("nv", _c_i),
+        ("precision", _c_i),
+        ("kx", ctypes.POINTER(_c_d)),
+        ("one_over_kx", ctypes.POINTER(_c_d)),
+        ("v", ctypes.POINTER(_c_d)),
+        ("kv", ctypes.POINTER(_c_d)),
+        ("dv", _c_d),
+        ("nu", _c_d),
+        ("dt", _c_d),
+        ("op", _c_i),
+    ]
+
+
+# name -> (argtypes); every function returns int status unless listed in _OTHER
+SIGNATURES = {
+    "vpfp_plan_create": [ctypes.POINTER(GridDesc), ctypes.POINTER(_c_p)],
+    "vpfp_plan_destroy": [_c_p],
+    "vpfp_vdfdx": [_c_p, _c_p, _c_p, _c_d, _c_p],
+    "vpfp_edfdv": [_c_p, _c_p, _c_p, _c_p, _c_d, _c_p],
+    "vpfp_compute_charges": [_c_p, _c_p, _c_p, _c_p],
+    "vpfp_solve_for_field": [_c_p, _c_p, _c_p, _c_p],
+    "vpfp_field": [_c_p, _c_p, _c_p, _c_p, _c_p],
+    "vpfp_collision_diagonals": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
+    "vpfp_tridiag_solve": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
+    "vpfp_collide": [_c_p, _c_p, _c_p, _c_p],
+    "vpfp_step_n": [_c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p],
+    "vpfp_last_error": [],
+    "vpfp_version": [],
+}
+_OTHER = {"vpfp_last_error": ctypes.c_char_p, "vpfp_version": _c_i}
+
+_lib = None
+
+
+def library():
+    """Load libvpfp.so once; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                "libvpfp.so is not built (expected %s); run `python -m vlapy_amd.build`" % LIB_PATH
+            )
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = _OTHER.get(name, _c_i)
+        _lib = lib
+    return _lib
+
+
+def check(status):
+    """Map a C ABI status to the reference's Python exceptions."""
+    if status == VPFP_OK:
+        return
+    msg = library().vpfp_last_error().decode(errors="replace")
+    if status == VPFP_EBADARG:
+        raise ValueError(msg)
+    if status == VPFP_EUNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise RuntimeError(msg)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(_c_d))
+
+
+class Plan:
+    """Owns one vpfp plan (device grid copies, twiddle tables, scratch)."""
+
+    def __init__(self, nx, nv, precision="fp64", kx=None, one_over_kx=None, v=None, kv=None, dv=1.0,
+                 nu=0.0, dt=0.0, op="lb"):
+        lib = library()
+        self.nx, self.nv = int(nx), int(nv)
+        self.precision = precision
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be fp64 or fp32")
+        if op not in ("lb", "dg"):
+            raise NotImplementedError("Collision Operator: <" + str(op) + "> has not yet been implemented")
+
+        def arr(a, n):
+            if a is None:
+                return np.zeros(n)
+            a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+            if a.shape != (n,):
+                raise ValueError("grid array has shape %s, expected (%d,)" % (a.shape, n))
+            return a
+
+        # keep the host copies alive for the duration of the call
+        self._kx, self._okx = arr(kx, self.nx), arr(one_over_kx, self.nx)
+        self._v, self._kv = arr(v, self.nv), arr(kv, self.nv)
+        desc = GridDesc(self.nx, self.nv, VPFP_F64 if precision == "fp64" else VPFP_F32,
+                        _dptr(self._kx), _dptr(self._okx), _dptr(self._v), _dptr(self._kv),
+                        float(dv), float(nu), float(dt), VPFP_OP_LB if op == "lb" else VPFP_OP_DG)
+        handle = _c_p()
+        check(lib.vpfp_plan_create(ctypes.byref(desc), ctypes.byref(handle)))
+        self.handle = handle
+        self.dv, self.nu, self.dt, self.op = float(dv), float(nu), float(dt), op
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.vpfp_plan_destroy(h)
+            self.handle = None

@@ -1,0 +1,613 @@
+// HIP kernels and the C ABI (include/vpfp.h) of the VPFP step on gfx950.
+//
+// Launch shapes (wave64):
+//   row kernel   M = nv/2 complex points per row, Q = min(8, M) per thread,
+//                TPT = M/Q threads per row, max(1, 64/TPT) rows per workgroup
+//   drift kernel M = nx/2, Q = min(8, M), TPT = M/Q threads per column,
+//                C columns per workgroup (16 for nx <= 256, 8 for nx = 512,
+//                4 above), one partial density row per workgroup
+//   field kernel one workgroup of 256 threads
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vpfp.h"
+#include "vpfp_kernels.h"
+
+using vp::cx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define VPFP_HIP(x)                                                                 \
+  do {                                                                              \
+    hipError_t e_ = (x);                                                            \
+    if (e_ != hipSuccess) return fail(VPFP_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+struct GpuCtx {
+  int tid, nthr, bid;
+  __device__ __forceinline__ void sync() { __syncthreads(); }
+  __device__ __forceinline__ double shfl_xor(double v, int m, int w) { return __shfl_xor(v, m, w); }
+  __device__ __forceinline__ float shfl_xor(float v, int m, int w) { return __shfl_xor(v, m, w); }
+};
+
+#define VPFP_CTX GpuCtx c{(int)threadIdx.x, (int)blockDim.x, (int)blockIdx.x}
+
+// ------------------------------------------------------------------ kernels
+template <class T, int M, int Q>
+__global__ __launch_bounds__((vp::RowCfg<T, M, Q>::wg)) void k_row(vp::RowArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  VPFP_CTX;
+  vp::row_body<T, M, Q>(c, lds, a);
+}
+
+template <class T, int M, int Q, int C>
+__global__ __launch_bounds__((vp::DriftCfg<T, M, Q, C>::wg)) void k_drift(vp::DriftArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  VPFP_CTX;
+  vp::drift_body<T, M, Q, C>(c, lds, a);
+}
+
+constexpr int kFT = 256;
+
+template <int M, int Q>
+__global__ __launch_bounds__(kFT) void k_field(vp::FieldArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  VPFP_CTX;
+  vp::field_body<M, Q, kFT>(c, lds, a);
+}
+
+__global__ __launch_bounds__(kFT) void k_field_dft(vp::FieldArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  VPFP_CTX;
+  vp::field_dft_body<kFT>(c, lds, a);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_density(const T* f, int nx, int nv, double scale, double* part) {
+  VPFP_CTX;
+  vp::density_body<T>(c, f, nx, nv, scale, part);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_diag(const T* f, int nx, int nv, const double* v, double dv, double nu_dt,
+                                              int op_dg, double* A, double* B, double* Cc) {
+  VPFP_CTX;
+  vp::diag_body<T>(c, f, nx, nv, v, dv, nu_dt, op_dg, A, B, Cc);
+}
+
+template <class T, int M, int Q>
+__global__ __launch_bounds__((vp::RowCfg<T, M, Q>::wg)) void k_tri(vp::TriArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  VPFP_CTX;
+  vp::tri_body<T, M, Q>(c, lds, a);
+}
+
+// ------------------------------------------------------------------ launch helpers
+int launched() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(VPFP_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return VPFP_OK;
+}
+
+template <class K>
+int set_lds(K kern, int bytes) {
+  if (bytes > 65536) {
+    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+    if (e != hipSuccess) return fail(VPFP_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  }
+  return VPFP_OK;
+}
+
+constexpr int qfor(int M) { return M < 8 ? M : 8; }
+
+template <class T, int M>
+int launch_row(const vp::RowArgs<T>& a, hipStream_t s) {
+  constexpr int Q = qfor(M);
+  using Cfg = vp::RowCfg<T, M, Q>;
+  auto kern = k_row<T, M, Q>;
+  if (int rc = set_lds(kern, Cfg::lds)) return rc;
+  const int grid = (a.nx + Cfg::rows - 1) / Cfg::rows;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(Cfg::wg), Cfg::lds, s, a);
+  return launched();
+}
+
+template <class T, int M>
+int launch_tri(const vp::TriArgs<T>& a, hipStream_t s) {
+  constexpr int Q = qfor(M);
+  using Cfg = vp::RowCfg<T, M, Q>;
+  auto kern = k_tri<T, M, Q>;
+  if (int rc = set_lds(kern, Cfg::lds)) return rc;
+  const int grid = (a.nx + Cfg::rows - 1) / Cfg::rows;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(Cfg::wg), Cfg::lds, s, a);
+  return launched();
+}
+
+template <class T, int M, int C>
+int launch_drift_c(const vp::DriftArgs<T>& a, hipStream_t s) {
+  constexpr int Q = qfor(M);
+  using Cfg = vp::DriftCfg<T, M, Q, C>;
+  auto kern = k_drift<T, M, Q, C>;
+  if (int rc = set_lds(kern, Cfg::lds)) return rc;
+  hipLaunchKernelGGL(kern, dim3(a.nv / C), dim3(Cfg::wg), Cfg::lds, s, a);
+  return launched();
+}
+
+template <class T, int M>
+int launch_drift(const vp::DriftArgs<T>& a, int C, hipStream_t s) {
+  switch (C) {
+    case 4: return launch_drift_c<T, M, 4>(a, s);
+    case 8: return launch_drift_c<T, M, 8>(a, s);
+    case 16: return launch_drift_c<T, M, 16>(a, s);
+  }
+  return fail(VPFP_EUNSUPPORTED, "drift tile width");
+}
+
+template <int M>
+int launch_field(const vp::FieldArgs& a, hipStream_t s) {
+  constexpr int Q = (M / kFT) > 2 ? (M / kFT) : 2;
+  using Cfg = vp::FieldCfg<M, Q, kFT>;
+  auto kern = k_field<M, Q>;
+  if (int rc = set_lds(kern, Cfg::lds)) return rc;
+  hipLaunchKernelGGL(kern, dim3(1), dim3(kFT), Cfg::lds, s, a);
+  return launched();
+}
+
+#define VPFP_SWITCH_M(M, CALL, MAXM)                 \
+  switch (M) {                                       \
+    case 2: return CALL(2);                          \
+    case 4: return CALL(4);                          \
+    case 8: return CALL(8);                          \
+    case 16: return CALL(16);                        \
+    case 32: return CALL(32);                        \
+    case 64: return CALL(64);                        \
+    case 128: return CALL(128);                      \
+    case 256: return CALL(256);                      \
+    case 512: return CALL(512);                      \
+    case 1024: if (MAXM >= 1024) return CALL(1024);  \
+      break;                                         \
+    case 2048: if (MAXM >= 2048) return CALL(2048);  \
+      break;                                         \
+    case 4096: if (MAXM >= 4096) return CALL(4096);  \
+      break;                                         \
+  }
+
+template <class T>
+int row_dispatch(const vp::RowArgs<T>& a, hipStream_t s) {
+  const int M = a.nv / 2;
+#define CALL(m) launch_row<T, ((m) <= 4096 ? (m) : 2)>(a, s)
+  VPFP_SWITCH_M(M, CALL, 4096)
+#undef CALL
+  return fail(VPFP_EUNSUPPORTED, "nv must be a power of two in [4, 8192]");
+}
+
+template <class T>
+int tri_dispatch(const vp::TriArgs<T>& a, hipStream_t s) {
+  const int M = a.nv / 2;
+#define CALL(m) launch_tri<T, m>(a, s)
+  VPFP_SWITCH_M(M, CALL, 4096)
+#undef CALL
+  return fail(VPFP_EUNSUPPORTED, "nv must be a power of two in [4, 8192]");
+}
+
+template <class T>
+int drift_dispatch(const vp::DriftArgs<T>& a, int C, hipStream_t s) {
+  const int M = a.nx / 2;
+#define CALL(m) launch_drift<T, ((m) <= 1024 ? (m) : 2)>(a, C, s)
+  VPFP_SWITCH_M(M, CALL, 1024)
+#undef CALL
+  return fail(VPFP_EUNSUPPORTED, "nx must be a power of two in [4, 2048] for the x-advection");
+}
+
+int field_dispatch(const vp::FieldArgs& a, bool pow2, hipStream_t s) {
+  if (pow2) {
+    const int M = a.nx / 2;
+#define CALL(m) launch_field<m>(a, s)
+    VPFP_SWITCH_M(M, CALL, 2048)
+#undef CALL
+  }
+  if (a.nx > 4096) return fail(VPFP_EUNSUPPORTED, "nx must be at most 4096 for the field solve");
+  const int lds = ((a.nx * 8 + 15) & ~15) + (a.nx > kFT ? a.nx : kFT) * 16 + 64;
+  if (int rc = set_lds(k_field_dft, lds)) return rc;
+  hipLaunchKernelGGL(k_field_dft, dim3(1), dim3(kFT), lds, s, a);
+  return launched();
+}
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// fftfreq table check: k[0] == 0, k[N-m] == -k[m], and k[M-m] == -k[M] - k[m]
+// to rounding (the phase identity used by PhaseMul)
+bool fftfreq_like(const double* k, int n) {
+  if (!is_pow2(n) || n < 4) return true;  // not used by the spectral kernels
+  const int M = n / 2;
+  if (k[0] != 0.0) return false;
+  const double scale = std::fabs(k[M]);
+  for (int m = 1; m < M; ++m) {
+    if (k[n - m] != -k[m]) return false;
+    if (std::fabs(k[M - m] - (-k[M] - k[m])) > 64.0 * 2.220446049250313e-16 * scale) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+struct vpfp_plan_s {
+  int nx = 0, nv = 0, prec = VPFP_F64, op = VPFP_OP_LB;
+  double dv = 0, nu = 0, dt = 0;
+  bool nx_pow2 = false, nv_pow2 = false;
+  int C = 0, ntiles = 0;
+  double *kx = nullptr, *okx = nullptr, *v = nullptr, *kv = nullptr;
+  void* tw_v = nullptr;          // cx<T>[nv]
+  void* tw_x = nullptr;          // cx<T>[nx]
+  cx<double>* tw_xd = nullptr;   // cx<double>[nx]
+  double* dens = nullptr;        // [max(ntiles,1)][nx]
+  double* ering = nullptr;       // [2][nx]
+  double* series = nullptr;      // [2]
+  std::vector<void*> allocs;
+  ~vpfp_plan_s() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+template <class P>
+int dalloc(vpfp_plan_s* p, P** out, size_t bytes) {
+  void* ptr = nullptr;
+  hipError_t e = hipMalloc(&ptr, bytes > 0 ? bytes : 16);
+  if (e != hipSuccess) return fail(VPFP_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  p->allocs.push_back(ptr);
+  *out = reinterpret_cast<P*>(ptr);
+  return VPFP_OK;
+}
+
+int upload(void* dst, const void* src, size_t bytes) {
+  VPFP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return VPFP_OK;
+}
+
+// exp(-2 pi i n / N) for n < N in long double, rounded
+template <class T>
+std::vector<cx<T>> twiddles(int N) {
+  std::vector<cx<T>> w(N);
+  const long double two_pi = 6.283185307179586476925286766559L;
+  for (int n = 0; n < N; ++n) {
+    const long double a = two_pi * (long double)n / (long double)N;
+    w[n] = vp::mk<T>((T)(double)cosl(a), (T)(double)(-sinl(a)));
+  }
+  return w;
+}
+
+template <class T>
+int make_tables(vpfp_plan_s* p) {
+  auto wv = twiddles<T>(p->nv);
+  auto wx = twiddles<T>(p->nx);
+  auto wxd = twiddles<double>(p->nx);
+  cx<T>* dv_ = nullptr;
+  cx<T>* dx_ = nullptr;
+  if (int rc = dalloc(p, &dv_, sizeof(cx<T>) * p->nv)) return rc;
+  if (int rc = dalloc(p, &dx_, sizeof(cx<T>) * p->nx)) return rc;
+  if (int rc = dalloc(p, &p->tw_xd, sizeof(cx<double>) * p->nx)) return rc;
+  if (int rc = upload(dv_, wv.data(), sizeof(cx<T>) * p->nv)) return rc;
+  if (int rc = upload(dx_, wx.data(), sizeof(cx<T>) * p->nx)) return rc;
+  if (int rc = upload(p->tw_xd, wxd.data(), sizeof(cx<double>) * p->nx)) return rc;
+  p->tw_v = dv_;
+  p->tw_x = dx_;
+  return VPFP_OK;
+}
+
+int drift_tile(int nx, int nv) {
+  const int M = nx / 2;
+  int C = M <= 128 ? 16 : (M == 256 ? 8 : 4);
+  while (C > nv) C /= 2;
+  return C >= 4 ? C : 0;
+}
+
+hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+template <class T>
+vp::RowArgs<T> row_args(vpfp_plan_t p, const void* fin, void* fout) {
+  vp::RowArgs<T> a;
+  std::memset(&a, 0, sizeof(a));
+  a.f_in = reinterpret_cast<const T*>(fin);
+  a.f_out = reinterpret_cast<T*>(fout);
+  a.nx = p->nx;
+  a.nv = p->nv;
+  a.kv = p->kv;
+  a.v = p->v;
+  a.dv = p->dv;
+  a.tw = reinterpret_cast<const cx<T>*>(p->tw_v);
+  a.nu_dt = p->nu * p->dt;
+  a.op_dg = p->op == VPFP_OP_DG;
+  return a;
+}
+
+template <class T>
+vp::DriftArgs<T> drift_args(vpfp_plan_t p, const void* fin, void* fout, double dt, double* dens) {
+  vp::DriftArgs<T> a;
+  a.f_in = reinterpret_cast<const T*>(fin);
+  a.f_out = reinterpret_cast<T*>(fout);
+  a.nx = p->nx;
+  a.nv = p->nv;
+  a.dt = dt;
+  a.kx = p->kx;
+  a.v = p->v;
+  a.tw = reinterpret_cast<const cx<T>*>(p->tw_x);
+  a.dens = dens;
+  return a;
+}
+
+vp::FieldArgs field_args(vpfp_plan_t p, const double* part, int ntiles, double scale, const double* driver,
+                         double* e_out, double* e_out2, double* series) {
+  vp::FieldArgs a;
+  a.part = part;
+  a.ntiles = ntiles;
+  a.nx = p->nx;
+  a.scale = scale;
+  a.okx = p->okx;
+  a.tw = p->tw_xd;
+  a.driver = driver;
+  a.e_out = e_out;
+  a.e_out2 = e_out2;
+  a.series = series;
+  return a;
+}
+
+int check_plan(vpfp_plan_t p) {
+  if (!p) return fail(VPFP_EBADARG, "null plan");
+  return VPFP_OK;
+}
+
+int need_spectral_v(vpfp_plan_t p) {
+  if (!p->nv_pow2 || p->nv < 4 || p->nv > 8192)
+    return fail(VPFP_EUNSUPPORTED, "the v-direction kernels need nv a power of two in [4, 8192], got " +
+                                       std::to_string(p->nv));
+  return VPFP_OK;
+}
+
+int need_spectral_x(vpfp_plan_t p) {
+  if (!p->nx_pow2 || p->nx < 4 || p->nx > 2048 || p->C == 0)
+    return fail(VPFP_EUNSUPPORTED, "the x-advection needs nx a power of two in [4, 2048] and nv >= 4, got nx=" +
+                                       std::to_string(p->nx));
+  return VPFP_OK;
+}
+
+template <class T>
+int do_edfdv(vpfp_plan_t p, const void* fin, void* fout, const double* e, double dt, hipStream_t s) {
+  auto a = row_args<T>(p, fin, fout);
+  a.e = e;
+  a.dt_a = dt;
+  a.flags = vp::RK_KICK_A;
+  return row_dispatch<T>(a, s);
+}
+
+template <class T>
+int do_vdfdx(vpfp_plan_t p, const void* fin, void* fout, double dt, double* dens, hipStream_t s) {
+  return drift_dispatch<T>(drift_args<T>(p, fin, fout, dt, dens), p->C, s);
+}
+
+template <class T>
+int do_density(vpfp_plan_t p, const void* f, double scale, double* part, hipStream_t s) {
+  const int rows = 4;
+  hipLaunchKernelGGL(k_density<T>, dim3((p->nx + rows - 1) / rows), dim3(64 * rows), 0, s,
+                     reinterpret_cast<const T*>(f), p->nx, p->nv, scale, part);
+  return launched();
+}
+
+template <class T>
+int do_collide(vpfp_plan_t p, const void* fin, void* fout, hipStream_t s) {
+  auto a = row_args<T>(p, fin, fout);
+  a.flags = vp::RK_COLLIDE;
+  return row_dispatch<T>(a, s);
+}
+
+template <class T>
+int do_diag(vpfp_plan_t p, const void* f, double* A, double* B, double* Cc, hipStream_t s) {
+  const int rows = 4;
+  hipLaunchKernelGGL(k_diag<T>, dim3((p->nx + rows - 1) / rows), dim3(64 * rows), 0, s,
+                     reinterpret_cast<const T*>(f), p->nx, p->nv, p->v, p->dv, p->nu * p->dt,
+                     (int)(p->op == VPFP_OP_DG), A, B, Cc);
+  return launched();
+}
+
+template <class T>
+int do_tri(vpfp_plan_t p, const double* A, const double* B, const double* Cc, const void* d, void* x,
+           hipStream_t s) {
+  vp::TriArgs<T> a;
+  a.a = A;
+  a.b = B;
+  a.cc = Cc;
+  a.d = reinterpret_cast<const T*>(d);
+  a.x = reinterpret_cast<T*>(x);
+  a.nx = p->nx;
+  a.nv = p->nv;
+  return tri_dispatch<T>(a, s);
+}
+
+template <class T>
+int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, double* e_hist, double* moments,
+              double* series, hipStream_t s) {
+  const bool coll = p->nu > 0.0;
+  const double half = 0.5 * p->dt;
+  auto ra = row_args<T>(p, f, f);
+  ra.e = e;
+  ra.dt_a = half;
+  ra.flags = vp::RK_KICK_A;
+  if (int rc = row_dispatch<T>(ra, s)) return rc;
+  const double* e_cur = e;
+  for (int i = 0; i < n; ++i) {
+    if (int rc = do_vdfdx<T>(p, f, f, p->dt, p->dens, s)) return rc;
+    double* ei = e_hist ? e_hist + (size_t)i * p->nx : p->ering + (size_t)(i & 1) * p->nx;
+    auto fa = field_args(p, p->dens, p->ntiles, p->dv, driver ? driver + (size_t)i * p->nx : nullptr, ei, nullptr,
+                         series ? series + 2 * (size_t)i : nullptr);
+    if (int rc = field_dispatch(fa, true, s)) return rc;
+    e_cur = ei;
+    auto rb = row_args<T>(p, f, f);
+    rb.e = e_cur;
+    rb.dt_a = half;
+    rb.dt_b = half;
+    rb.moments = moments ? moments + (size_t)i * vp::kMoments * p->nx : nullptr;
+    rb.flags = vp::RK_KICK_A | (coll ? vp::RK_COLLIDE : 0) | (moments ? vp::RK_MOMENTS : 0) |
+               (i + 1 < n ? vp::RK_KICK_B : 0);
+    if (int rc = row_dispatch<T>(rb, s)) return rc;
+  }
+  if (n > 0) VPFP_HIP(hipMemcpyAsync(e, e_cur, sizeof(double) * p->nx, hipMemcpyDeviceToDevice, s));
+  return VPFP_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+int vpfp_version(void) { return 1; }
+
+const char* vpfp_last_error(void) { return g_err.c_str(); }
+
+int vpfp_plan_create(const vpfp_grid_desc* d, vpfp_plan_t* out) {
+  if (!d || !out) return fail(VPFP_EBADARG, "null argument");
+  *out = nullptr;
+  if (d->nx < 2 || d->nv < 2) return fail(VPFP_EBADARG, "nx and nv must be at least 2");
+  if (d->precision != VPFP_F64 && d->precision != VPFP_F32) return fail(VPFP_EBADARG, "unknown precision");
+  if (d->op != VPFP_OP_LB && d->op != VPFP_OP_DG)
+    return fail(VPFP_EUNSUPPORTED, "collision operator must be lb or dg");
+  if (d->nu < 0.0) return fail(VPFP_EUNSUPPORTED, "negative collision frequency");
+  if (!d->kx || !d->one_over_kx || !d->v || !d->kv) return fail(VPFP_EBADARG, "null grid array");
+  if (!(d->dv > 0.0)) return fail(VPFP_EBADARG, "dv must be positive");
+  if (!fftfreq_like(d->kx, d->nx)) return fail(VPFP_EUNSUPPORTED, "kx is not an fftfreq wavenumber table");
+  if (!fftfreq_like(d->kv, d->nv)) return fail(VPFP_EUNSUPPORTED, "kv is not an fftfreq wavenumber table");
+
+  auto* p = new vpfp_plan_s();
+  p->nx = d->nx;
+  p->nv = d->nv;
+  p->prec = d->precision;
+  p->op = d->op;
+  p->dv = d->dv;
+  p->nu = d->nu;
+  p->dt = d->dt;
+  p->nx_pow2 = is_pow2(d->nx);
+  p->nv_pow2 = is_pow2(d->nv);
+  p->C = (p->nx_pow2 && p->nv_pow2 && p->nx >= 4) ? drift_tile(p->nx, p->nv) : 0;
+  p->ntiles = p->C ? p->nv / p->C : 0;
+  int rc = VPFP_OK;
+  const size_t nxb = sizeof(double) * d->nx, nvb = sizeof(double) * d->nv;
+  if (!rc) rc = dalloc(p, &p->kx, nxb);
+  if (!rc) rc = dalloc(p, &p->okx, nxb);
+  if (!rc) rc = dalloc(p, &p->v, nvb);
+  if (!rc) rc = dalloc(p, &p->kv, nvb);
+  if (!rc) rc = upload(p->kx, d->kx, nxb);
+  if (!rc) rc = upload(p->okx, d->one_over_kx, nxb);
+  if (!rc) rc = upload(p->v, d->v, nvb);
+  if (!rc) rc = upload(p->kv, d->kv, nvb);
+  if (!rc) rc = p->prec == VPFP_F64 ? make_tables<double>(p) : make_tables<float>(p);
+  if (!rc) rc = dalloc(p, &p->dens, nxb * (size_t)(p->ntiles > 1 ? p->ntiles : 1));
+  if (!rc) rc = dalloc(p, &p->ering, 2 * nxb);
+  if (!rc) rc = dalloc(p, &p->series, 2 * sizeof(double));
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  *out = p;
+  return VPFP_OK;
+}
+
+int vpfp_plan_destroy(vpfp_plan_t p) {
+  if (!p) return VPFP_OK;
+  (void)hipDeviceSynchronize();
+  delete p;
+  return VPFP_OK;
+}
+
+int vpfp_vdfdx(vpfp_plan_t p, const void* fin, void* fout, double dt, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (int rc = need_spectral_x(p)) return rc;
+  if (!fin || !fout) return fail(VPFP_EBADARG, "null array");
+  return p->prec == VPFP_F64 ? do_vdfdx<double>(p, fin, fout, dt, nullptr, S(stream))
+                             : do_vdfdx<float>(p, fin, fout, dt, nullptr, S(stream));
+}
+
+int vpfp_edfdv(vpfp_plan_t p, const void* fin, void* fout, const double* e, double dt, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (int rc = need_spectral_v(p)) return rc;
+  if (!fin || !fout || !e) return fail(VPFP_EBADARG, "null array");
+  return p->prec == VPFP_F64 ? do_edfdv<double>(p, fin, fout, e, dt, S(stream))
+                             : do_edfdv<float>(p, fin, fout, e, dt, S(stream));
+}
+
+int vpfp_compute_charges(vpfp_plan_t p, const void* f, double* n_out, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!f || !n_out) return fail(VPFP_EBADARG, "null array");
+  return p->prec == VPFP_F64 ? do_density<double>(p, f, p->dv, n_out, S(stream))
+                             : do_density<float>(p, f, p->dv, n_out, S(stream));
+}
+
+int vpfp_solve_for_field(vpfp_plan_t p, const double* n, double* e_out, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!n || !e_out) return fail(VPFP_EBADARG, "null array");
+  if (p->nx < 2) return fail(VPFP_EUNSUPPORTED, "nx must be at least 2");
+  auto fa = field_args(p, n, 1, 1.0, nullptr, e_out, nullptr, nullptr);
+  return field_dispatch(fa, p->nx_pow2 && p->nx >= 4 && p->nx <= 4096, S(stream));
+}
+
+int vpfp_field(vpfp_plan_t p, const void* f, const double* driver, double* e_out, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!f || !e_out) return fail(VPFP_EBADARG, "null array");
+  int rc = p->prec == VPFP_F64 ? do_density<double>(p, f, p->dv, p->dens, S(stream))
+                               : do_density<float>(p, f, p->dv, p->dens, S(stream));
+  if (rc) return rc;
+  auto fa = field_args(p, p->dens, 1, 1.0, driver, e_out, nullptr, nullptr);
+  return field_dispatch(fa, p->nx_pow2 && p->nx >= 4 && p->nx <= 4096, S(stream));
+}
+
+int vpfp_collision_diagonals(vpfp_plan_t p, const void* f, double* a, double* b, double* c, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!f || !a || !b || !c) return fail(VPFP_EBADARG, "null array");
+  return p->prec == VPFP_F64 ? do_diag<double>(p, f, a, b, c, S(stream)) : do_diag<float>(p, f, a, b, c, S(stream));
+}
+
+int vpfp_tridiag_solve(vpfp_plan_t p, const double* a, const double* b, const double* c, const void* d, void* x,
+                       void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (int rc = need_spectral_v(p)) return rc;
+  if (!a || !b || !c || !d || !x) return fail(VPFP_EBADARG, "null array");
+  return p->prec == VPFP_F64 ? do_tri<double>(p, a, b, c, d, x, S(stream))
+                             : do_tri<float>(p, a, b, c, d, x, S(stream));
+}
+
+int vpfp_collide(vpfp_plan_t p, const void* fin, void* fout, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!fin || !fout) return fail(VPFP_EBADARG, "null array");
+  if (p->nu == 0.0) {
+    if (fin != fout) {
+      const size_t bytes = (size_t)p->nx * p->nv * (p->prec == VPFP_F64 ? 8 : 4);
+      VPFP_HIP(hipMemcpyAsync(fout, fin, bytes, hipMemcpyDeviceToDevice, S(stream)));
+    }
+    return VPFP_OK;
+  }
+  if (int rc = need_spectral_v(p)) return rc;
+  return p->prec == VPFP_F64 ? do_collide<double>(p, fin, fout, S(stream)) : do_collide<float>(p, fin, fout, S(stream));
+}
+
+int vpfp_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, double* e_hist, double* moments,
+                double* series, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!f || !e) return fail(VPFP_EBADARG, "null array");
+  if (n < 0) return fail(VPFP_EBADARG, "negative step count");
+  if (int rc = need_spectral_v(p)) return rc;
+  if (int rc = need_spectral_x(p)) return rc;
+  if (p->nx > 2048) return fail(VPFP_EUNSUPPORTED, "nx too large for the fused step");
+  return p->prec == VPFP_F64 ? do_step_n<double>(p, f, e, n, driver, e_hist, moments, series, S(stream))
+                             : do_step_n<float>(p, f, e, n, driver, e_hist, moments, series, S(stream));
+}
+
+}  // extern "C"

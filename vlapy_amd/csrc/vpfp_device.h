@@ -1,0 +1,421 @@
+// Device building blocks for the VPFP kernels (gfx950, wave64).
+//
+// Everything here is written against a small block context `Ctx` that
+// provides `tid` (thread index in the workgroup), `sync()` (workgroup
+// barrier) and `shfl_xor(v, mask, width)` (lane exchange inside a
+// wavefront).  The HIP kernels in vpfp.hip instantiate it with the hardware
+// primitives.
+//
+// Contents
+//   * cx<T>        : complex value in registers / LDS (T = double or float)
+//   * dft<R>       : R-point DFT in registers (radix-2 network, R <= 32)
+//   * fft<M,TPT>   : M-point complex FFT of one sequence held by TPT threads,
+//                    Q = M/TPT values per thread in registers, exchanged
+//                    through a padded LDS buffer between Stockham passes
+//   * couple()     : half-length real-FFT unpack, Fourier multiplier, repack
+//   * row_sum<>    : sum over the TPT threads that own one sequence
+//   * tridiag      : partition solver for one diagonally dominant
+//                    tridiagonal system held by TPT threads (chunk sweeps,
+//                    parallel cyclic reduction of the 2*TPT interface
+//                    unknowns in LDS, Thomas on the chunk interiors)
+#pragma once
+
+#if defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define VP_HD __host__ __device__ __forceinline__
+#else
+#include <cmath>
+#define VP_HD inline
+#endif
+
+namespace vp {
+
+// ---------------------------------------------------------------------------
+// complex arithmetic
+// ---------------------------------------------------------------------------
+template <class T>
+struct cx {
+  T re, im;
+};
+
+template <class T>
+VP_HD cx<T> mk(T r, T i) {
+  cx<T> z;
+  z.re = r;
+  z.im = i;
+  return z;
+}
+template <class T>
+VP_HD cx<T> add(cx<T> a, cx<T> b) { return mk<T>(a.re + b.re, a.im + b.im); }
+template <class T>
+VP_HD cx<T> sub(cx<T> a, cx<T> b) { return mk<T>(a.re - b.re, a.im - b.im); }
+template <class T>
+VP_HD cx<T> mul(cx<T> a, cx<T> b) {
+  return mk<T>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+// a * conj(b)
+template <class T>
+VP_HD cx<T> mulc(cx<T> a, cx<T> b) {
+  return mk<T>(a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im);
+}
+
+// Padded LDS index: one spare slot every 16 elements keeps the strided
+// writes of the first Stockham pass and the chunk reads of the tridiagonal
+// solver spread over the LDS banks.
+VP_HD constexpr int pidx(int i) { return i + (i >> 4); }
+template <int M>
+struct Padded {
+  static constexpr int size = M + ((M - 1) >> 4) + 1;
+};
+
+// ---------------------------------------------------------------------------
+// exp(-i theta) in the working precision
+// ---------------------------------------------------------------------------
+VP_HD void cis_neg(double th, cx<double>& out) {
+  double s, c;
+  sincos(th, &s, &c);
+  out = mk<double>(c, -s);
+}
+VP_HD void cis_neg(double th, cx<float>& out) {
+  // reduce in double so that large arguments keep float accuracy
+  const double r = th - 6.283185307179586 * rint(th * 0.15915494309189535);
+  float s, c;
+  sincosf((float)r, &s, &c);
+  out = mk<float>(c, -s);
+}
+
+// ---------------------------------------------------------------------------
+// R-point DFT in registers (radix-2 decimation in time, R <= 32)
+// ---------------------------------------------------------------------------
+// cos(2 pi j / 32) for j = 0..8, correctly rounded
+VP_HD constexpr double cos32q(int j) {
+  return j == 0   ? 1.0
+         : j == 1 ? 0.9807852804032304
+         : j == 2 ? 0.9238795325112867
+         : j == 3 ? 0.8314696123025452
+         : j == 4 ? 0.7071067811865476
+         : j == 5 ? 0.5555702330196022
+         : j == 6 ? 0.3826834323650898
+         : j == 7 ? 0.19509032201612828
+                  : 0.0;
+}
+VP_HD constexpr double cos32(int J) {
+  return (J & 31) <= 8    ? cos32q(J & 31)
+         : (J & 31) <= 16 ? -cos32q(16 - (J & 31))
+         : (J & 31) <= 24 ? -cos32q((J & 31) - 16)
+                          : cos32q(32 - (J & 31));
+}
+VP_HD constexpr double sin32(int J) { return cos32(J - 8); }
+
+// b * W_32^J  (forward: W = exp(-2 pi i J / 32); inverse: conjugate)
+template <bool INV, class T>
+VP_HD cx<T> tw32(cx<T> b, int J) {
+  J &= 31;
+  if (J == 0) return b;
+  if (J == 16) return mk<T>(-b.re, -b.im);
+  if (J == 8) return INV ? mk<T>(-b.im, b.re) : mk<T>(b.im, -b.re);
+  if (J == 24) return INV ? mk<T>(b.im, -b.re) : mk<T>(-b.im, b.re);
+  const T c = (T)cos32(J), s = (T)sin32(J);
+  return INV ? mk<T>(b.re * c - b.im * s, b.im * c + b.re * s)
+             : mk<T>(b.re * c + b.im * s, b.im * c - b.re * s);
+}
+
+template <int R>
+VP_HD constexpr int brev(int i) {
+  int r = 0;
+  for (int b = 1; b < R; b <<= 1) {
+    r = (r << 1) | (i & 1);
+    i >>= 1;
+  }
+  return r;
+}
+
+template <int R, bool INV, class T>
+VP_HD void dft(cx<T>* x) {
+  static_assert(R >= 1 && R <= 32 && (R & (R - 1)) == 0, "radix must be a power of two <= 32");
+  if constexpr (R == 1) return;
+  cx<T> y[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) y[brev<R>(i)] = x[i];
+#pragma unroll
+  for (int h = 1; h < R; h <<= 1) {
+#pragma unroll
+    for (int g = 0; g < R; g += 2 * h) {
+#pragma unroll
+      for (int j = 0; j < h; ++j) {
+        const cx<T> a = y[g + j];
+        const cx<T> b = tw32<INV>(y[g + j + h], j * (16 / h));
+        y[g + j] = add(a, b);
+        y[g + j + h] = sub(a, b);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < R; ++i) x[i] = y[i];
+}
+
+// ---------------------------------------------------------------------------
+// M-point complex FFT held by TPT threads (Stockham autosort, radix <= 16)
+//
+// Register layout: thread t holds element t + i*TPT in v[i], i < Q = M/TPT.
+// A pass of radix R with NS = product of the earlier radices takes the
+// butterfly j = t + q*TPT (inputs j + r*M/R = v[q + r*Q/R]), twiddles input r
+// by W_{NS*R}^{r*(j mod NS)}, does an R-point DFT and writes output r to
+// position (j - j mod NS)*R + j mod NS + r*NS.  After the last pass the data
+// are in natural order, both in v and in `buf`.
+//
+// tw is a table of exp(-2 pi i n / (M*twmul)), n < M*twmul.
+// ---------------------------------------------------------------------------
+// `act` false: the thread only takes part in the barriers (no LDS writes).
+template <class T, int M, int TPT, int R, int NS, bool INV, class Ctx>
+VP_HD void st_pass(Ctx& c, cx<T>* v, cx<T>* buf, const cx<T>* tw, int twmul, int t, bool act) {
+  constexpr int Q = M / TPT;
+  constexpr int NB = Q / R;
+  static_assert(NB * R == Q, "radix must divide the per-thread count");
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int j = t + q * TPT;
+    const int k = j & (NS - 1);
+    cx<T> u[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) u[r] = v[q + r * NB];
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const cx<T> w = tw[(r * k * (M / (NS * R))) * twmul];
+        u[r] = INV ? mulc(u[r], w) : mul(u[r], w);
+      }
+    }
+    dft<R, INV>(u);
+    const int base = (j - k) * R + k;
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[pidx(base + r * NS)] = u[r];
+    }
+  }
+  c.sync();
+#pragma unroll
+  for (int i = 0; i < Q; ++i) v[i] = buf[pidx(t + i * TPT)];
+  c.sync();
+}
+
+template <class T, int M, int TPT, bool INV, int NS = 1, class Ctx>
+VP_HD void fft(Ctx& c, cx<T>* v, cx<T>* buf, const cx<T>* tw, int twmul, int t, bool act = true) {
+  constexpr int Q = M / TPT;
+  static_assert(Q >= 2 && Q * TPT == M, "each thread must hold at least two values");
+  constexpr int RMAX = Q < 16 ? Q : 16;
+  constexpr int R = (M / NS) < RMAX ? (M / NS) : RMAX;
+  st_pass<T, M, TPT, R, NS, INV>(c, v, buf, tw, twmul, t, act);
+  if constexpr (NS * R < M) fft<T, M, TPT, INV, NS * R>(c, v, buf, tw, twmul, t, act);
+}
+
+// ---------------------------------------------------------------------------
+// Fourier multipliers for the half-length real transform
+//
+// For own bin k in [0, M) `pair` returns P_k and P_{M-k}; for k == 0 it
+// returns the real factors applied to the DC and Nyquist bins (the real part
+// of the multiplier, which is what taking Re() of the complex inverse leaves).
+// ---------------------------------------------------------------------------
+
+// Phase shift exp(-i (kvec[k]*dt)*s) of the exponential pushers.  kvec is an
+// fftfreq wavenumber table (checked on the host: kvec[N-m] == -kvec[m] and
+// linear in m), so P_{M-k} = exp(-i theta_M) * conj(P_k) with
+// theta_M = (-kvec[M]*dt)*s the positive Nyquist phase.
+template <class T>
+struct PhaseMul {
+  const double* kvec;
+  double dt, s;
+  cx<T> pm;  // exp(-i theta_M)
+  VP_HD PhaseMul(const double* kvec_, double dt_, double s_, double kvec_nyq) : kvec(kvec_), dt(dt_), s(s_) {
+    cis_neg((-kvec_nyq * dt) * s, pm);
+  }
+  VP_HD void pair(int k, cx<T>& pk, cx<T>& pmk) const {
+    if (k == 0) {
+      pk = mk<T>(T(1), T(0));
+      pmk = mk<T>(pm.re, T(0));
+      return;
+    }
+    cis_neg((kvec[k] * dt) * s, pk);
+    pmk = mulc(pm, pk);
+  }
+};
+
+// Poisson multiplier i * one_over_kx (zero real part at DC and Nyquist).
+template <class T>
+struct PoissonMul {
+  const double* okx;
+  int M;
+  VP_HD void pair(int k, cx<T>& pk, cx<T>& pmk) const {
+    if (k == 0) {
+      pk = mk<T>(T(0), T(0));
+      pmk = mk<T>(T(0), T(0));
+      return;
+    }
+    pk = mk<T>(T(0), (T)okx[k]);
+    pmk = mk<T>(T(0), (T)okx[M - k]);
+  }
+};
+
+// z_j = f_{2j} + i f_{2j+1} was transformed to Z (M points).  With
+// E_k = (Z_k + conj Z_{M-k})/2, O_k = (Z_k - conj Z_{M-k})/(2i) and
+// W = exp(-2 pi i k / N), the real N-point spectrum is F_k = E_k + W O_k and
+// F_{k+M} = E_k - W O_k.  After F'_k = P_k F_k, F'_{k+M} = conj(P_{M-k}) F_{k+M}
+// the repacked spectrum Z'_k = (F'_k + F'_{k+M} + i (F'_k - F'_{k+M}) conj W)/N
+// inverse-transforms (unnormalised M-point) to f'_{2j} + i f'_{2j+1}.
+// v[i] holds Z_k and pz[i] holds Z_{M-k} for k = t + i*TPT; twN[k] = W^k.
+template <class T, int M, int TPT, class Mul>
+VP_HD void couple(cx<T>* v, const cx<T>* pz, const cx<T>* twN, int t, const Mul& mult) {
+  constexpr int Q = M / TPT;
+  const T invN = T(1) / T(2 * M);
+  const T h = T(0.5);
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int k = t + i * TPT;
+    const cx<T> a = v[i], p = pz[i];
+    const cx<T> E = mk<T>((a.re + p.re) * h, (a.im - p.im) * h);
+    const cx<T> O = mk<T>((a.im + p.im) * h, (p.re - a.re) * h);
+    const cx<T> W = twN[k];
+    const cx<T> WO = mul(W, O);
+    const cx<T> Fk = add(E, WO), Fkm = sub(E, WO);
+    cx<T> pk, pmk;
+    mult.pair(k, pk, pmk);
+    const cx<T> G1 = mul(pk, Fk);
+    const cx<T> G2 = mulc(Fkm, pmk);
+    const cx<T> S = add(G1, G2);
+    const cx<T> D = mulc(sub(G1, G2), W);
+    v[i] = mk<T>((S.re - D.im) * invN, (S.im + D.re) * invN);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// sums over the TPT threads that own one sequence (all of them get the sum)
+// scratch: (TPT/64)*NV doubles per sequence when TPT > 64 (one sequence per
+// workgroup in that case)
+// ---------------------------------------------------------------------------
+template <int TPT, int NV, class Ctx>
+VP_HD void row_sum(Ctx& c, double* vals, double* scratch) {
+  constexpr int W = TPT < 64 ? TPT : 64;
+#pragma unroll
+  for (int m = 1; m < W; m <<= 1) {
+#pragma unroll
+    for (int q = 0; q < NV; ++q) vals[q] += c.shfl_xor(vals[q], m, W);
+  }
+  if constexpr (TPT > 64) {
+    constexpr int NW = TPT / 64;
+    const int w = c.tid >> 6;
+    if ((c.tid & 63) == 0) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) scratch[w * NV + q] = vals[q];
+    }
+    c.sync();
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      double s = 0.0;
+#pragma unroll
+      for (int ww = 0; ww < NW; ++ww) s += scratch[ww * NV + q];
+      vals[q] = s;
+    }
+    c.sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Partition solver for one tridiagonal system of N = TPT*L unknowns.
+//
+// Row m reads  l(m) x_{m-1} + b(m) x_m + u(m) x_{m+1} = d_m  with l(0) = 0 and
+// u(N-1) = 0.  Thread t owns rows m0 = t*L .. m0+L-1 with d in `d[L]`; on
+// return `d` holds the solution.  Coef provides l(m), u(m), b(m).
+// red: 8*TPT doubles of LDS for the interface system (one system per
+// 2*TPT interface rows).
+// ---------------------------------------------------------------------------
+template <int TPT, int L, class Coef, class Ctx>
+VP_HD void tridiag(Ctx& c, double* d, const Coef& cf, int t, double* red) {
+  static_assert(L >= 2, "chunk must hold at least two rows");
+  const int m0 = t * L;
+  // sweep down keeping x_{m0} free: x_i = p + q x_{m0} - r x_{i+1}
+  double p = 0, q = 0, r = 0;
+#pragma unroll
+  for (int i = 1; i < L; ++i) {
+    const int m = m0 + i;
+    const double lm = cf.l(m), um = cf.u(m), bm = cf.b(m);
+    const double den = (i == 1) ? bm : bm - lm * r;
+    const double inv = 1.0 / den;
+    p = (i == 1) ? d[i] * inv : (d[i] - lm * p) * inv;
+    q = (i == 1) ? -lm * inv : -lm * q * inv;
+    r = um * inv;
+  }
+  // sweep up keeping x_{m0+L-1} free: x_i = p2 + q2 x_{m0+L-1} - r2 x_{i-1}
+  double p2 = 0, q2 = 0, r2 = 0;
+#pragma unroll
+  for (int i = L - 2; i >= 0; --i) {
+    const int m = m0 + i;
+    const double lm = cf.l(m), um = cf.u(m), bm = cf.b(m);
+    const double den = (i == L - 2) ? bm : bm - um * r2;
+    const double inv = 1.0 / den;
+    p2 = (i == L - 2) ? d[i] * inv : (d[i] - um * p2) * inv;
+    q2 = (i == L - 2) ? -um * inv : -um * q2 * inv;
+    r2 = lm * inv;
+  }
+  // interface system: y_{2t} = x_{m0}, y_{2t+1} = x_{m0+L-1}
+  //   row 2t  :  r2 y_{2t-1} + y_{2t} - q2 y_{2t+1} = p2
+  //   row 2t+1: -q  y_{2t}   + y_{2t+1} + r  y_{2t+2} = p
+  constexpr int NR = 2 * TPT;
+  double ra[2] = {r2, -q}, rb[2] = {1.0, 1.0}, rc[2] = {-q2, r}, rd[2] = {p2, p};
+  double* A = red;
+  double* B = red + NR;
+  double* C = red + 2 * NR;
+  double* D = red + 3 * NR;
+  for (int s = 1; s < NR; s <<= 1) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = 2 * t + h;
+      A[i] = ra[h];
+      B[i] = rb[h];
+      C[i] = rc[h];
+      D[i] = rd[h];
+    }
+    c.sync();
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = 2 * t + h;
+      const int im = i - s, ip = i + s;
+      double na = 0.0, nb = rb[h], nc = 0.0, nd = rd[h];
+      if (im >= 0) {
+        const double al = -ra[h] / B[im];
+        na = al * A[im];
+        nb += al * C[im];
+        nd += al * D[im];
+      }
+      if (ip < NR) {
+        const double ga = -rc[h] / B[ip];
+        nc = ga * C[ip];
+        nb += ga * A[ip];
+        nd += ga * D[ip];
+      }
+      ra[h] = na;
+      rb[h] = nb;
+      rc[h] = nc;
+      rd[h] = nd;
+    }
+    c.sync();
+  }
+  const double xs = rd[0] / rb[0];
+  const double xe = rd[1] / rb[1];
+  // interior rows m0+1 .. m0+L-2 with both chunk ends known (Thomas)
+  double rr[L];
+  d[0] = xs;
+#pragma unroll
+  for (int i = 1; i < L - 1; ++i) {
+    const int m = m0 + i;
+    const double lm = cf.l(m), um = cf.u(m), bm = cf.b(m);
+    const double den = (i == 1) ? bm : bm - lm * rr[i - 1];
+    const double inv = 1.0 / den;
+    d[i] = (i == 1) ? (d[i] - lm * xs) * inv : (d[i] - lm * d[i - 1]) * inv;
+    rr[i] = um * inv;
+  }
+  d[L - 1] = xe;
+#pragma unroll
+  for (int i = L - 2; i >= 1; --i) d[i] = d[i] - rr[i] * d[i + 1];
+}
+
+}  // namespace vp

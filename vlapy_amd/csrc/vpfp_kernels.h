@@ -1,0 +1,531 @@
+// Kernel bodies of the VPFP step.  Each body is written for one workgroup
+// and a block context `Ctx` (tid, nthr, bid, sync, shfl_xor); vpfp.hip wraps
+// them in __global__ kernels.
+//
+// Data layout (HBM): f is (nx, nv) row-major, v fastest, T = double or float.
+// A row (fixed x) is contiguous; a column (fixed v) has stride nv.
+//
+//   row_body    one or more x-rows per workgroup.  Optional stages, in order:
+//               kick A (v-advection by e(x) dt_a), collision (implicit LB/DG
+//               solve along v), storage moments, kick B (v-advection by e(x)
+//               dt_b).  One HBM read and one write of the row whatever the
+//               stages.   [vlasov.py:113-140, collisions.py:26-160,222-265,
+//               step.py:143-228]
+//   drift_body  C v-columns per workgroup: x-advection of each column and the
+//               per-column-tile partial trapz sums sum_v w_v f(x, v) feeding the
+//               Poisson solve.   [vlasov.py:83-110, field.py:27-36]
+//   field_body  one workgroup: density from the partial sums, pseudo-spectral
+//               Poisson solve, driver added, E written (and <E^2>, <D^2>).
+//               [field.py:39-88, step.py:215-216]
+#pragma once
+#include "vpfp_device.h"
+
+namespace vp {
+
+enum RowFlags { RK_KICK_A = 1, RK_COLLIDE = 2, RK_MOMENTS = 4, RK_KICK_B = 8 };
+constexpr int kMoments = 8;  // n, j, T, q, fv4, vN, f^2, f log f
+
+// trapz weight of sample m of an N-point row
+VP_HD double trapw(int m, int N) { return (m == 0 || m == N - 1) ? 0.5 : 1.0; }
+
+// ---------------------------------------------------------------------------
+// spectral multiply along a length-2M real sequence held as M complex values
+// ---------------------------------------------------------------------------
+template <class T, int M, int TPT, class Mul, class Ctx>
+VP_HD void spectral_apply(Ctx& c, cx<T>* v, cx<T>* buf, const cx<T>* tw, int t, const Mul& mult,
+                          bool act = true) {
+  constexpr int Q = M / TPT;
+  fft<T, M, TPT, false>(c, v, buf, tw, 2, t, act);
+  cx<T> pz[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) pz[i] = buf[pidx((M - (t + i * TPT)) & (M - 1))];
+  c.sync();
+  couple<T, M, TPT>(v, pz, tw, t, mult);
+  fft<T, M, TPT, true>(c, v, buf, tw, 2, t, act);
+}
+
+// ---------------------------------------------------------------------------
+// collision coefficients (collisions.py:67-79 LB, :138-156 DG)
+//   l(m) = nu dt (-T/dv^2 + (v_{m-1} - vbar)/(2 dv)),  m >= 1
+//   u(m) = nu dt (-T/dv^2 - (v_{m+1} - vbar)/(2 dv)),  m <= N-2
+//   b    = 1 + nu dt (2 T / dv^2)
+// ---------------------------------------------------------------------------
+struct CollCoef {
+  const double* v;
+  int N;
+  double ndt, A0, H, vbar, bd;
+  VP_HD double l(int m) const { return m == 0 ? 0.0 : ndt * (A0 + (v[m - 1] - vbar) * H); }
+  VP_HD double u(int m) const { return m == N - 1 ? 0.0 : ndt * (A0 - (v[m + 1] - vbar) * H); }
+  VP_HD double b(int) const { return bd; }
+};
+
+// explicit diagonals a (nv-1), b (nv), c (nv-1) of one row
+struct ArrCoef {
+  const double* a;
+  const double* bb;
+  const double* cc;
+  int N;
+  VP_HD double l(int m) const { return m == 0 ? 0.0 : a[m - 1]; }
+  VP_HD double u(int m) const { return m == N - 1 ? 0.0 : cc[m]; }
+  VP_HD double b(int m) const { return bb[m]; }
+};
+
+// drift velocity and temperature of one row (un-normalised, trapz), chunk layout
+template <int TPT, int L, class Ctx>
+VP_HD void coll_moments(Ctx& c, const double* d, const double* v, int N, double dv, bool dg, int t,
+                        double* rs, double& vbar, double& temp) {
+  const int m0 = t * L;
+  double s[1] = {0.0};
+  vbar = 0.0;
+  if (dg) {
+#pragma unroll
+    for (int i = 0; i < L; ++i) s[0] += trapw(m0 + i, N) * d[i] * v[m0 + i];
+    row_sum<TPT, 1>(c, s, rs);
+    vbar = dv * s[0];
+    s[0] = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < L; ++i) {
+    const double w = v[m0 + i] - vbar;
+    s[0] += trapw(m0 + i, N) * d[i] * (w * w);
+  }
+  row_sum<TPT, 1>(c, s, rs);
+  temp = dv * s[0];
+}
+
+// ---------------------------------------------------------------------------
+// row kernel
+// ---------------------------------------------------------------------------
+template <class T>
+struct RowArgs {
+  const T* f_in;
+  T* f_out;
+  int nx, nv;
+  const double* e;  // [nx] field for the kicks
+  double dt_a, dt_b;
+  const double* kv;  // [nv]
+  const double* v;   // [nv]
+  double dv;
+  const cx<T>* tw;  // exp(-2 pi i n / nv), n < nv
+  double nu_dt;
+  int op_dg;
+  double* moments;  // [kMoments][nx]
+  int flags;
+};
+
+template <class T, int M, int Q>
+struct RowCfg {
+  static constexpr int TPT = M / Q;
+  static constexpr int L = 2 * Q;
+  static constexpr int fft_bytes = Padded<M>::size * (int)sizeof(cx<T>);
+  static constexpr int red_bytes = 8 * TPT * 8;
+  static constexpr int region = ((fft_bytes > red_bytes ? fft_bytes : red_bytes) + 15) & ~15;
+  static constexpr int wg = TPT >= 64 ? TPT : 64;
+  static constexpr int rows = wg / TPT;
+  static constexpr int sum_bytes = (TPT > 64 ? TPT / 64 : 1) * kMoments * 8;
+  static constexpr int lds = rows * region + sum_bytes;
+};
+
+template <class T, int M, int Q, class Ctx>
+VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
+  using Cfg = RowCfg<T, M, Q>;
+  constexpr int TPT = Cfg::TPT;
+  constexpr int L = Cfg::L;
+  const int t = c.tid % TPT;
+  const int rl = c.tid / TPT;
+  const int row = c.bid * Cfg::rows + rl;
+  const bool valid = row < a.nx;
+  const int rowc = valid ? row : a.nx - 1;
+  cx<T>* buf = reinterpret_cast<cx<T>*>(lds + rl * Cfg::region);
+  double* red = reinterpret_cast<double*>(lds + rl * Cfg::region);
+  double* rs = reinterpret_cast<double*>(lds + Cfg::rows * Cfg::region);
+
+  const cx<T>* src = reinterpret_cast<const cx<T>*>(a.f_in + (size_t)rowc * a.nv);
+  cx<T> v[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) v[i] = src[t + i * TPT];
+
+  const double e = (a.flags & (RK_KICK_A | RK_KICK_B)) ? a.e[rowc] : 0.0;
+  if (a.flags & RK_KICK_A)
+    spectral_apply<T, M, TPT>(c, v, buf, a.tw, t, PhaseMul<T>(a.kv, a.dt_a, e, a.kv[M]));
+
+  if (a.flags & (RK_COLLIDE | RK_MOMENTS)) {
+    // natural order -> contiguous chunk of L samples per thread
+#pragma unroll
+    for (int i = 0; i < Q; ++i) buf[pidx(t + i * TPT)] = v[i];
+    c.sync();
+    double d[L];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const cx<T> z = buf[pidx(t * Q + i)];
+      d[2 * i] = (double)z.re;
+      d[2 * i + 1] = (double)z.im;
+    }
+    c.sync();
+    const int N = a.nv;
+    const int m0 = t * L;
+    if ((a.flags & RK_COLLIDE) && a.nu_dt > 0.0) {
+      double vbar, temp;
+      coll_moments<TPT, L>(c, d, a.v, N, a.dv, a.op_dg != 0, t, rs, vbar, temp);
+      const double dv2 = a.dv * a.dv;
+      CollCoef cf;
+      cf.v = a.v;
+      cf.N = N;
+      cf.ndt = a.nu_dt;
+      cf.A0 = -temp / dv2;
+      cf.H = 0.5 / a.dv;
+      cf.vbar = vbar;
+      cf.bd = 1.0 + a.nu_dt * (2.0 * temp / dv2);
+      tridiag<TPT, L>(c, d, cf, t, red);
+    }
+    if (a.flags & RK_MOMENTS) {
+      double s[kMoments];
+#pragma unroll
+      for (int q = 0; q < kMoments; ++q) s[q] = 0.0;
+#pragma unroll
+      for (int i = 0; i < L; ++i) {
+        const int m = m0 + i;
+        const double wf = trapw(m, N) * d[i];
+        const double vm = a.v[m];
+        double p = wf;
+        s[0] += p;
+        p *= vm;
+        s[1] += p;
+        p *= vm;
+        s[2] += p;
+        p *= vm;
+        s[3] += p;
+        p *= vm;
+        s[4] += p;
+        p *= vm;
+        s[5] += p;
+        s[6] += wf * d[i];
+        s[7] += wf * log(d[i]);
+      }
+      row_sum<TPT, kMoments>(c, s, rs);
+      if (valid && t == 0) {
+#pragma unroll
+        for (int q = 0; q < kMoments; ++q) a.moments[(size_t)q * a.nx + row] = a.dv * s[q];
+      }
+    }
+    // chunk -> natural order
+#pragma unroll
+    for (int i = 0; i < Q; ++i) buf[pidx(t * Q + i)] = mk<T>((T)d[2 * i], (T)d[2 * i + 1]);
+    c.sync();
+#pragma unroll
+    for (int i = 0; i < Q; ++i) v[i] = buf[pidx(t + i * TPT)];
+    c.sync();
+  }
+
+  if (a.flags & RK_KICK_B)
+    spectral_apply<T, M, TPT>(c, v, buf, a.tw, t, PhaseMul<T>(a.kv, a.dt_b, e, a.kv[M]));
+
+  if (valid) {
+    cx<T>* dst = reinterpret_cast<cx<T>*>(a.f_out + (size_t)row * a.nv);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) dst[t + i * TPT] = v[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// drift kernel (x-advection of C columns) + partial density sums
+// ---------------------------------------------------------------------------
+template <class T>
+struct DriftArgs {
+  const T* f_in;
+  T* f_out;
+  int nx, nv;
+  double dt;
+  const double* kx;  // [nx]
+  const double* v;   // [nv]
+  const cx<T>* tw;   // exp(-2 pi i n / nx), n < nx
+  double* dens;      // [nv / C][nx] partial sums (unscaled), or null
+};
+
+template <class T, int M, int Q, int C>
+struct DriftCfg {
+  static constexpr int TPT = M / Q;
+  static constexpr int wg = TPT * C;
+  static constexpr int stride = Padded<M>::size;
+  static constexpr int lds = C * stride * (int)sizeof(cx<T>);
+};
+
+template <class T, int M, int Q, int C, class Ctx>
+VP_HD void drift_body(Ctx& c, unsigned char* lds, const DriftArgs<T>& a) {
+  using Cfg = DriftCfg<T, M, Q, C>;
+  constexpr int TPT = Cfg::TPT;
+  const int cc = c.tid % C;
+  const int t = c.tid / C;
+  const int col = c.bid * C + cc;
+  cx<T>* buf = reinterpret_cast<cx<T>*>(lds) + cc * Cfg::stride;
+  const size_t nv = (size_t)a.nv;
+  const T* fi = a.f_in + col;
+  cx<T> v[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int j = t + i * TPT;
+    v[i] = mk<T>(fi[(size_t)(2 * j) * nv], fi[(size_t)(2 * j + 1) * nv]);
+  }
+  spectral_apply<T, M, TPT>(c, v, buf, a.tw, t, PhaseMul<T>(a.kx, a.dt, a.v[col], a.kx[M]));
+  T* fo = a.f_out + col;
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int j = t + i * TPT;
+    fo[(size_t)(2 * j) * nv] = v[i].re;
+    fo[(size_t)(2 * j + 1) * nv] = v[i].im;
+  }
+  if (a.dens) {
+    const double w = trapw(col, a.nv);
+    double* out = a.dens + (size_t)c.bid * a.nx;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      double s[2] = {w * (double)v[i].re, w * (double)v[i].im};
+#pragma unroll
+      for (int m = 1; m < C; m <<= 1) {
+        s[0] += c.shfl_xor(s[0], m, C);
+        s[1] += c.shfl_xor(s[1], m, C);
+      }
+      if (cc == 0) {
+        const int j = t + i * TPT;
+        out[2 * j] = s[0];
+        out[2 * j + 1] = s[1];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// field kernel: n(x) = scale * sum_tiles part[tile][x];  E = Re ifft(i/kx fft(1 - n)) + driver
+// ---------------------------------------------------------------------------
+struct FieldArgs {
+  const double* part;
+  int ntiles;
+  int nx;
+  double scale;
+  const double* okx;      // [nx]
+  const cx<double>* tw;   // exp(-2 pi i n / nx), n < nx
+  const double* driver;   // [nx] or null
+  double* e_out;          // [nx]
+  double* e_out2;         // [nx] or null
+  double* series;         // [2] or null: mean(E^2), mean(driver^2)
+};
+
+// reduce the partial sums into rho[x] = 1 - scale * sum (all FT threads)
+template <int FT, class Ctx>
+VP_HD void field_density(Ctx& c, const FieldArgs& a, double* rho, double* tmp) {
+  const int nx = a.nx;
+  const int G = nx >= FT ? 1 : FT / nx;
+  for (int x0 = c.tid; x0 < nx * G; x0 += FT) {
+    const int x = x0 % nx, g = x0 / nx;
+    double s = 0.0;
+    for (int tile = g; tile < a.ntiles; tile += G) s += a.part[(size_t)tile * nx + x];
+    tmp[x0] = s;
+  }
+  c.sync();
+  for (int x = c.tid; x < nx; x += FT) {
+    double s = 0.0;
+    for (int g = 0; g < G; ++g) s += tmp[g * nx + x];
+    rho[x] = 1.0 - a.scale * s;
+  }
+  c.sync();
+}
+
+template <int FT, class Ctx>
+VP_HD void field_finish(Ctx& c, const FieldArgs& a, int x, double ex, double* rs, double* acc) {
+  const double dr = a.driver ? a.driver[x] : 0.0;
+  const double e = ex + dr;
+  a.e_out[x] = e;
+  if (a.e_out2) a.e_out2[x] = e;
+  acc[0] += e * e;
+  acc[1] += dr * dr;
+}
+
+template <int M, int Q, int FT>
+struct FieldCfg {
+  static constexpr int TPT = M / Q;
+  static constexpr int nx = 2 * M;
+  static constexpr int tmp_n = (nx > FT ? nx : FT);
+  static constexpr int lds = nx * 8 + ((Padded<M>::size * 16 > tmp_n * 8) ? Padded<M>::size * 16 : tmp_n * 8) + 64;
+};
+
+template <int M, int Q, int FT, class Ctx>
+VP_HD void field_body(Ctx& c, unsigned char* lds, const FieldArgs& a) {
+  using Cfg = FieldCfg<M, Q, FT>;
+  constexpr int TPT = Cfg::TPT;
+  static_assert(TPT <= FT, "field transform needs at most FT threads");
+  double* rho = reinterpret_cast<double*>(lds);
+  cx<double>* buf = reinterpret_cast<cx<double>*>(lds + Cfg::nx * 8);
+  double* tmp = reinterpret_cast<double*>(buf);
+  double* rs = reinterpret_cast<double*>(lds + Cfg::lds - 64);
+  field_density<FT>(c, a, rho, tmp);
+  const bool act = c.tid < TPT;
+  const int t = act ? c.tid : 0;
+  cx<double> v[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int j = t + i * TPT;
+    v[i] = mk<double>(rho[2 * j], rho[2 * j + 1]);
+  }
+  PoissonMul<double> mul;
+  mul.okx = a.okx;
+  mul.M = M;
+  spectral_apply<double, M, TPT>(c, v, buf, a.tw, t, mul, act);
+  double acc[2] = {0.0, 0.0};
+  if (act) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int j = t + i * TPT;
+      field_finish<FT>(c, a, 2 * j, v[i].re, rs, acc);
+      field_finish<FT>(c, a, 2 * j + 1, v[i].im, rs, acc);
+    }
+  }
+  if (a.series) {
+    row_sum<FT, 2>(c, acc, rs);
+    if (c.tid == 0) {
+      a.series[0] = acc[0] / a.nx;
+      a.series[1] = acc[1] / a.nx;
+    }
+  }
+}
+
+// direct-DFT field solve for any nx (used when nx is not a power of two)
+template <int FT, class Ctx>
+VP_HD void field_dft_body(Ctx& c, unsigned char* lds, const FieldArgs& a) {
+  const int nx = a.nx;
+  double* rho = reinterpret_cast<double*>(lds);
+  cx<double>* F = reinterpret_cast<cx<double>*>(lds + ((nx * 8 + 15) & ~15));
+  double* tmp = reinterpret_cast<double*>(F);
+  double* rs = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(F) + (size_t)(nx > FT ? nx : FT) * 16);
+  field_density<FT>(c, a, rho, tmp);
+  for (int m = c.tid; m < nx; m += FT) {
+    cx<double> s = mk<double>(0.0, 0.0);
+    for (int j = 0; j < nx; ++j) {
+      const cx<double> w = a.tw[(int)(((long long)m * j) % nx)];
+      s.re += rho[j] * w.re;
+      s.im += rho[j] * w.im;
+    }
+    // effective multiplier Re-projection: (G_m + conj G_{nx-m}) / 2, G = i okx
+    const double g = 0.5 * (a.okx[m] - a.okx[(nx - m) % nx]);
+    F[m] = mk<double>(-g * s.im, g * s.re);
+  }
+  c.sync();
+  double acc[2] = {0.0, 0.0};
+  for (int x = c.tid; x < nx; x += FT) {
+    double s = 0.0;
+    for (int m = 0; m < nx; ++m) {
+      const cx<double> w = a.tw[(int)(((long long)m * x) % nx)];
+      // Re(F_m conj(w))
+      s += F[m].re * w.re + F[m].im * w.im;
+    }
+    field_finish<FT>(c, a, x, s / nx, rs, acc);
+  }
+  if (a.series) {
+    row_sum<FT, 2>(c, acc, rs);
+    if (c.tid == 0) {
+      a.series[0] = acc[0] / a.nx;
+      a.series[1] = acc[1] / a.nx;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// density rows: part[x] = scale * sum_v w_v f(x, v)   (one wavefront per row)
+// ---------------------------------------------------------------------------
+template <class T, class Ctx>
+VP_HD void density_body(Ctx& c, const T* f, int nx, int nv, double scale, double* part) {
+  const int lane = c.tid & 63;
+  const int row = c.bid * (c.nthr / 64) + (c.tid >> 6);
+  const int rowc = row < nx ? row : nx - 1;
+  const T* fr = f + (size_t)rowc * nv;
+  double s[1] = {0.0};
+  for (int m = lane; m < nv; m += 64) s[0] += trapw(m, nv) * (double)fr[m];
+#pragma unroll
+  for (int k = 1; k < 64; k <<= 1) s[0] += c.shfl_xor(s[0], k, 64);
+  if (row < nx && lane == 0) part[row] = scale * s[0];
+}
+
+// ---------------------------------------------------------------------------
+// collision diagonals a, b, c of every row (one wavefront per row)
+// ---------------------------------------------------------------------------
+template <class T, class Ctx>
+VP_HD void diag_body(Ctx& c, const T* f, int nx, int nv, const double* v, double dv, double nu_dt, int op_dg,
+                     double* A, double* B, double* Cc) {
+  const int lane = c.tid & 63;
+  const int row = c.bid * (c.nthr / 64) + (c.tid >> 6);
+  const int rowc = row < nx ? row : nx - 1;
+  const T* fr = f + (size_t)rowc * nv;
+  double vbar = 0.0;
+  double s[1] = {0.0};
+  if (op_dg) {
+    for (int m = lane; m < nv; m += 64) s[0] += trapw(m, nv) * (double)fr[m] * v[m];
+#pragma unroll
+    for (int k = 1; k < 64; k <<= 1) s[0] += c.shfl_xor(s[0], k, 64);
+    vbar = dv * s[0];
+    s[0] = 0.0;
+  }
+  for (int m = lane; m < nv; m += 64) {
+    const double w = v[m] - vbar;
+    s[0] += trapw(m, nv) * (double)fr[m] * (w * w);
+  }
+#pragma unroll
+  for (int k = 1; k < 64; k <<= 1) s[0] += c.shfl_xor(s[0], k, 64);
+  const double temp = dv * s[0];
+  if (row >= nx) return;
+  const double dv2 = dv * dv;
+  const double A0 = -temp / dv2, H = 0.5 / dv;
+  const double bd = 1.0 + nu_dt * (2.0 * temp / dv2);
+  double* ar = A + (size_t)row * (nv - 1);
+  double* cr = Cc + (size_t)row * (nv - 1);
+  double* br = B + (size_t)row * nv;
+  for (int m = lane; m < nv; m += 64) {
+    br[m] = bd;
+    if (m < nv - 1) {
+      ar[m] = nu_dt * (A0 + (v[m] - vbar) * H);
+      cr[m] = nu_dt * (A0 - (v[m + 1] - vbar) * H);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// tridiagonal solve with explicit diagonals (batched over rows)
+// ---------------------------------------------------------------------------
+template <class T>
+struct TriArgs {
+  const double* a;  // [nx][nv-1]
+  const double* b;  // [nx][nv]
+  const double* cc; // [nx][nv-1]
+  const T* d;       // [nx][nv]
+  T* x;             // [nx][nv]
+  int nx, nv;
+};
+
+template <class T, int M, int Q, class Ctx>
+VP_HD void tri_body(Ctx& c, unsigned char* lds, const TriArgs<T>& a) {
+  using Cfg = RowCfg<T, M, Q>;
+  constexpr int TPT = Cfg::TPT;
+  constexpr int L = Cfg::L;
+  const int t = c.tid % TPT;
+  const int rl = c.tid / TPT;
+  const int row = c.bid * Cfg::rows + rl;
+  const bool valid = row < a.nx;
+  const int rowc = valid ? row : a.nx - 1;
+  double* red = reinterpret_cast<double*>(lds + rl * Cfg::region);
+  const int N = a.nv;
+  const T* dr = a.d + (size_t)rowc * N;
+  double d[L];
+#pragma unroll
+  for (int i = 0; i < L; ++i) d[i] = (double)dr[t * L + i];
+  ArrCoef cf;
+  cf.a = a.a + (size_t)rowc * (N - 1);
+  cf.bb = a.b + (size_t)rowc * N;
+  cf.cc = a.cc + (size_t)rowc * (N - 1);
+  cf.N = N;
+  tridiag<TPT, L>(c, d, cf, t, red);
+  if (valid) {
+    T* xr = a.x + (size_t)row * N;
+#pragma unroll
+    for (int i = 0; i < L; ++i) xr[t * L + i] = (T)d[i];
+  }
+}
+
+}  // namespace vp
