@@ -1,0 +1,254 @@
+#!/usr/bin/env python
+"""Benchmark of the fused VPFP time step on one MI355X.
+
+Workload (the collisional nonlinear electron plasma wave): nx=64 cells on
+[0, 2 pi/k), k=0.3; nv=1024 cells on [-6.4, 6.4]; f0 = (1 + 0.05 cos kx)
+exp(-v^2/2)/sqrt(2 pi); e0 = 0; Dougherty collisions nu=1e-3 every step;
+dt=0.05; 4000 leapfrog steps.  The GPU runs it with f stored and advected in
+FP32 and the velocity moments, Poisson solve and tridiagonal elimination in
+FP64; the plain FP64 path is timed beside it.  A step is the full reference
+timestep: kick, drift, field, kick, collision, storage moments (E history,
+n, j, T, q, v^4, v^5, f^2, f log f per x, <E^2>).
+
+    python bench.py [--steps 4000] [--warmup 200]
+
+Prints one JSON line.  `value` is phase-space cell-updates per second
+(nx * nv * steps / wall time of the timed region, inputs resident in HBM).
+"""
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+METRIC = "VPFP cell-updates/s + % HBM roofline; scan runs/s at ≤1% damping-rate error"
+HBM_PEAK_GBS = 8000.0
+
+NX, NV, K0, VMAX, AMP, NU, DT, NSTEPS = 64, 1024, 0.3, 6.4, 0.05, 1e-3, 0.05, 4000
+WORKLOAD = ("collisional nonlinear EPW: nx=64, nv=1024, v in [-6.4, 6.4], k=0.3, density 1+0.05cos(kx) "
+            "on a unit Maxwellian, Dougherty nu=1e-3 implicit per x every step, dt=0.05, 4000 steps")
+
+
+def initial_state(np):
+    xmax = 2 * np.pi / K0
+    dx = xmax / NX
+    x = np.linspace(dx / 2.0, xmax - dx / 2.0, NX)
+    kx = np.fft.fftfreq(NX, d=dx) * 2.0 * np.pi
+    okx = np.zeros_like(kx)
+    okx[1:] = 1.0 / kx[1:]
+    dv = 2 * VMAX / NV
+    v = np.linspace(-VMAX + dv / 2.0, VMAX - dv / 2.0, NV)
+    kv = np.fft.fftfreq(NV, d=dv) * 2.0 * np.pi
+    f0 = (1.0 + AMP * np.cos(K0 * x))[:, None] * (np.exp(-v ** 2 / 2.0) / np.sqrt(2 * np.pi))[None, :]
+    return x, kx, okx, v, kv, dv, f0
+
+
+# --------------------------------------------------------------------------- CPU baseline
+
+
+def cpu_child(target_s):
+    """Time the NumPy FP64 oracle on one core (run in a child with 1 BLAS/OMP thread)."""
+    import numpy as np
+
+    sys.path.insert(0, HERE)
+    from oracle import vpfp_oracle as O
+
+    x, kx, okx, v, kv, dv, f = initial_state(np)
+    g = O.Grid(x, kx, okx, v, kv, dv)
+    e = np.zeros(NX)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        e, f = O.leapfrog_step(g, e, f, steps * DT, DT)
+        f = O.collide(f, v, NU, DT, dv, "dg")
+        mom = O.step_moments(f, v, dv)
+        O.step_series(mom, e, np.zeros(NX), f, dv)
+        steps += 1
+        el = time.perf_counter() - t0
+        if (el >= target_s and steps >= 10) or steps >= NSTEPS:
+            break
+    print(json.dumps({"steps": steps, "seconds": el}))
+
+
+def cpu_baseline(target_s):
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
+        env[k] = "1"
+    res = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", str(target_s)],
+                         capture_output=True, text=True, env=env, timeout=600)
+    if res.returncode != 0:
+        raise RuntimeError("cpu baseline failed: " + res.stderr[-2000:])
+    r = json.loads(res.stdout.strip().splitlines()[-1])
+    return {
+        "value": NX * NV * r["steps"] / r["seconds"],
+        "unit": "cell-updates/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "first %d of the 4000 steps of the same workload on the NumPy FP64 oracle "
+                  "(leapfrog + Dougherty collision + storage moments), %.1f s, one thread" % (r["steps"], r["seconds"]),
+    }
+
+
+# --------------------------------------------------------------------------- GPU
+
+
+def gpu_run(precision, steps, warmup, profile_steps):
+    import numpy as np
+    import torch
+
+    from vlapy_amd import ops
+    from vlapy_amd._lib import NMOMENTS, Plan
+
+    dev = torch.device("cuda:0")
+    x, kx, okx, v, kv, dv, f0 = initial_state(np)
+    plan = Plan(NX, NV, precision, kx, okx, v, kv, dv, NU, DT, "dg")
+    fdt = ops.f_dtype(plan)
+
+    def fresh():
+        return (torch.as_tensor(f0, dtype=fdt, device=dev).contiguous(),
+                torch.zeros(NX, dtype=torch.float64, device=dev))
+
+    def outputs(n):
+        return (torch.zeros(n, NX, dtype=torch.float64, device=dev),
+                torch.zeros(n, NMOMENTS, NX, dtype=torch.float64, device=dev),
+                torch.zeros(n, 2, dtype=torch.float64, device=dev))
+
+    # warmup (untimed)
+    f, e = fresh()
+    if warmup > 0:
+        eh, mo, se = outputs(warmup)
+        ops.step_n(plan, f, e, warmup, None, eh, mo, se)
+    torch.cuda.synchronize()
+
+    # timed region: `steps` full timesteps from the initial state
+    f, e = fresh()
+    eh, mo, se = outputs(steps)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    ops.step_n(plan, f, e, steps, None, eh, mo, se)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = start.elapsed_time(stop)
+    if not torch.isfinite(eh).all():
+        raise RuntimeError("non-finite field history")
+    result = {"ms": ms, "wall_s": wall, "e_hist": eh.cpu().numpy(), "series": se.cpu().numpy(),
+              "f": f.double().cpu().numpy()}
+
+    # per-kernel durations from HIP events around every launch (separate pass)
+    f, e = fresh()
+    eh, mo, se = outputs(profile_steps)
+    plan.set_profiling(True)
+    ops.step_n(plan, f, e, profile_steps, None, eh, mo, se)
+    torch.cuda.synchronize()
+    result["kernels"] = plan.kernel_times()
+    plan.set_profiling(False)
+    result["bytes_f"] = 8 if precision == "fp64" else 4
+    result["ntiles"] = NV // 16
+    return result
+
+
+def roofline(run):
+    """Dominant kernel: achieved algorithmic GB/s vs the 8 TB/s HBM peak."""
+    cells = NX * NV
+    s = run["bytes_f"]
+    kern = run["kernels"]
+    dom = max(("row", "drift", "field"), key=lambda k: kern[k][0])
+    total_ms, launches = kern[dom]
+    avg_s = total_ms / launches / 1e3
+    # algorithmic bytes per launch: one read and one write of f (plus the
+    # partial densities for the drift; E and moments for the row kernel)
+    alg = {
+        "row": 2 * s * cells + 8 * NX * 9,
+        "drift": 2 * s * cells + 8 * NX * run["ntiles"],
+        "field": 8 * NX * run["ntiles"] + 8 * NX,
+    }[dom]
+    achieved = alg / avg_s / 1e9
+    traffic = None
+    prof = os.path.join(HERE, "profiles", "traffic_c2_%s.json" % ("fp64" if s == 8 else "fp32"))
+    if os.path.exists(prof):
+        with open(prof) as fh:
+            traffic = json.load(fh).get(dom)
+    return {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "alg_bytes_per_launch": alg,
+        "avg_launch_us": avg_s * 1e6,
+        "kernel_us": {k: (v[0] / v[1] * 1e3 if v[1] else None) for k, v in kern.items()},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=NSTEPS)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--profile-steps", type=int, default=1000)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-child", type=float, default=None, help=argparse.SUPPRESS)
+    args = ap.parse_args()
+
+    if args.cpu_child is not None:
+        cpu_child(args.cpu_child)
+        return 0
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the build has no multi-GPU path (instances of the "
+              "parameter scan shard with no exchange; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
+    r32 = gpu_run("fp32", args.steps, args.warmup, min(args.profile_steps, args.steps))
+    r64 = gpu_run("fp64", args.steps, args.warmup, min(args.profile_steps, args.steps))
+
+    import numpy as np
+
+    cells = NX * NV
+    value = cells * args.steps / (r32["ms"] / 1e3)
+    e32, e64 = r32["e_hist"], r64["e_hist"]
+    err_e = float(np.abs(e32 - e64).max() / np.abs(e64).max())
+    err_f = float(np.abs(r32["f"] - r64["f"]).max() / np.abs(r64["f"]).max())
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": WORKLOAD,
+            "precision": "f32 storage and advection FFTs; f64 moments, Poisson and tridiagonal",
+            "grid": [NX, NV],
+            "parallelism": "single GPU",
+        },
+        "roofline": roofline(r32),
+        "cpu_baseline": cpu,
+        "fp64": {
+            "value": cells * args.steps / (r64["ms"] / 1e3),
+            "ms_per_step": r64["ms"] / args.steps,
+            "roofline": roofline(r64),
+        },
+        "f32_vs_f64_after_run": {"E_max_rel": err_e, "f_max_rel": err_f},
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -113,6 +113,16 @@ int vpfp_collide(vpfp_plan_t plan, const void* f_in, void* f_out, void* stream);
 int vpfp_step_n(vpfp_plan_t plan, void* f, double* e, int n_steps, const double* driver, double* e_hist,
                 double* moments, double* series, void* stream);
 
+/* Instrumentation (no counterpart in the reference, which only logs wall
+ * time per step to mlflow, manager.py:105-172).  With profiling on,
+ * vpfp_step_n brackets each kernel launch with HIP events on its stream and
+ * waits for them before returning; vpfp_plan_kernel_times returns the summed
+ * milliseconds and launch counts per kernel class since profiling was
+ * switched on: [0] row (kick/collide/moments), [1] drift (x-advection +
+ * partial density), [2] field, [3] other. */
+int vpfp_plan_set_profiling(vpfp_plan_t plan, int on);
+int vpfp_plan_kernel_times(vpfp_plan_t plan, double* ms /*[4]*/, long long* counts /*[4]*/);
+
 const char* vpfp_last_error(void);
 int vpfp_version(void);
 

@@ -69,7 +69,7 @@ def test_edfdv(dev, nx, nv, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("nx,nv", [s for s in SIZES if s[0] >= 4])
+@pytest.mark.parametrize("nx,nv", [s for s in SIZES if s[0] >= 4] + [(1024, 8192), (2048, 16)])
 def test_vdfdx(dev, nx, nv, prec):
     from vlapy_amd import ops
 
@@ -159,13 +159,17 @@ def test_diagonals_and_tridiag(dev, op):
     assert rel(x.cpu().numpy(), O.thomas(wa, wb, wc, f)) < F64_TOL
 
 
-@pytest.mark.parametrize("nu,op", [(0.0, "lb"), (1e-2, "dg"), (1e-2, "lb")])
-def test_step_n_matches_oracle_loop(dev, nu, op):
+@pytest.mark.parametrize("nx,nv,nu,op,n", [(16, 64, 0.0, "lb", 12), (16, 64, 1e-2, "dg", 12), (16, 64, 1e-2, "lb", 12),
+                                          (256, 64, 1e-2, "dg", 4), (1024, 1024, 1e-3, "dg", 2)])
+def test_step_n_matches_oracle_loop(dev, nx, nv, nu, op, n):
+    """Fused step vs the oracle's leapfrog + collision loop; nx >= 128 runs the
+    field solve on several workgroups, nx = 1024 the drift on column-tile
+    strides."""
     import torch
     from vlapy_amd import ops
 
-    nx, nv, k, dt, n = 16, 64, 0.35, 0.1, 12
-    print("step_n", nu, op, flush=True)
+    k, dt = 0.35, 0.1
+    print("step_n", nx, nv, nu, op, flush=True)
     g = O.Grid.make(nx, nv, 2 * np.pi / k, 6.0)
     f0 = O.perturbed_maxwellian(g.x, g.v, k, 0.2)
     rng = np.random.default_rng(3)

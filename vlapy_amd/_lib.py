@@ -53,6 +53,8 @@ SIGNATURES = {
     "vpfp_tridiag_solve": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
     "vpfp_collide": [_c_p, _c_p, _c_p, _c_p],
     "vpfp_step_n": [_c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p],
+    "vpfp_plan_set_profiling": [_c_p, _c_i],
+    "vpfp_plan_kernel_times": [_c_p, ctypes.POINTER(_c_d), ctypes.POINTER(ctypes.c_longlong)],
     "vpfp_last_error": [],
     "vpfp_version": [],
 }
@@ -125,6 +127,16 @@ class Plan:
         check(lib.vpfp_plan_create(ctypes.byref(desc), ctypes.byref(handle)))
         self.handle = handle
         self.dv, self.nu, self.dt, self.op = float(dv), float(nu), float(dt), op
+
+    def set_profiling(self, on):
+        check(library().vpfp_plan_set_profiling(self.handle, 1 if on else 0))
+
+    def kernel_times(self):
+        """{kind: (total_ms, launches)} for kinds row, drift, field, other."""
+        ms = (_c_d * 4)()
+        n = (ctypes.c_longlong * 4)()
+        check(library().vpfp_plan_kernel_times(self.handle, ms, n))
+        return {k: (ms[i], n[i]) for i, k in enumerate(("row", "drift", "field", "other"))}
 
     def __del__(self):
         h = getattr(self, "handle", None)

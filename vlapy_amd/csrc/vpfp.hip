@@ -35,13 +35,19 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 struct GpuCtx {
-  int tid, nthr, bid;
+  int tid, nthr, bid, nblk;
   __device__ __forceinline__ void sync() { __syncthreads(); }
+  __device__ __forceinline__ void fence() { __threadfence(); }
+  __device__ __forceinline__ unsigned arrive(unsigned* c) { return atomicAdd(c, 1u); }
+  __device__ __forceinline__ void reset(unsigned* c) { __hip_atomic_store(c, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+  __device__ __forceinline__ double load_coherent(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
   __device__ __forceinline__ double shfl_xor(double v, int m, int w) { return __shfl_xor(v, m, w); }
   __device__ __forceinline__ float shfl_xor(float v, int m, int w) { return __shfl_xor(v, m, w); }
 };
 
-#define VPFP_CTX GpuCtx c{(int)threadIdx.x, (int)blockDim.x, (int)blockIdx.x}
+#define VPFP_CTX GpuCtx c{(int)threadIdx.x, (int)blockDim.x, (int)blockIdx.x, (int)gridDim.x}
 
 // ------------------------------------------------------------------ kernels
 template <class T, int M, int Q>
@@ -111,6 +117,13 @@ int set_lds(K kern, int bytes) {
 
 constexpr int qfor(int M) { return M < 8 ? M : 8; }
 
+// drift workgroups: one per column tile up to 1024, then grid-stride
+constexpr int kDriftMaxWG = 1024;
+int drift_grid(int ntiles) { return ntiles < kDriftMaxWG ? ntiles : kDriftMaxWG; }
+
+// field workgroups: one per 64 x values (the last to finish solves Poisson)
+int field_grid(int nx) { return nx >= 128 ? nx / 64 : 1; }
+
 template <class T, int M>
 int launch_row(const vp::RowArgs<T>& a, hipStream_t s) {
   constexpr int Q = qfor(M);
@@ -139,7 +152,7 @@ int launch_drift_c(const vp::DriftArgs<T>& a, hipStream_t s) {
   using Cfg = vp::DriftCfg<T, M, Q, C>;
   auto kern = k_drift<T, M, Q, C>;
   if (int rc = set_lds(kern, Cfg::lds)) return rc;
-  hipLaunchKernelGGL(kern, dim3(a.nv / C), dim3(Cfg::wg), Cfg::lds, s, a);
+  hipLaunchKernelGGL(kern, dim3(drift_grid(a.nv / C)), dim3(Cfg::wg), Cfg::lds, s, a);
   return launched();
 }
 
@@ -155,11 +168,11 @@ int launch_drift(const vp::DriftArgs<T>& a, int C, hipStream_t s) {
 
 template <int M>
 int launch_field(const vp::FieldArgs& a, hipStream_t s) {
-  constexpr int Q = (M / kFT) > 2 ? (M / kFT) : 2;
+  constexpr int Q = (M / kFT) > 8 ? (M / kFT) : (M < 8 ? M : 8);
   using Cfg = vp::FieldCfg<M, Q, kFT>;
   auto kern = k_field<M, Q>;
   if (int rc = set_lds(kern, Cfg::lds)) return rc;
-  hipLaunchKernelGGL(kern, dim3(1), dim3(kFT), Cfg::lds, s, a);
+  hipLaunchKernelGGL(kern, dim3(field_grid(a.nx)), dim3(kFT), Cfg::lds, s, a);
   return launched();
 }
 
@@ -225,17 +238,30 @@ int field_dispatch(const vp::FieldArgs& a, bool pow2, hipStream_t s) {
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
-// fftfreq table check: k[0] == 0, k[N-m] == -k[m], and k[M-m] == -k[M] - k[m]
-// to rounding (the phase identity used by PhaseMul)
+// fftfreq table check: k[0] == 0, k[N-m] == -k[m], and k[m] == m k[1] to
+// rounding for m <= N/2 (k[N/2] negative): the kernels form the phase
+// k[m] * dt * s as m * (k[1] * dt * s)
 bool fftfreq_like(const double* k, int n) {
   if (!is_pow2(n) || n < 4) return true;  // not used by the spectral kernels
   const int M = n / 2;
   if (k[0] != 0.0) return false;
-  const double scale = std::fabs(k[M]);
+  const double tol = 64.0 * 2.220446049250313e-16 * std::fabs(M * k[1]);
+  if (std::fabs(k[M] + M * k[1]) > tol) return false;
   for (int m = 1; m < M; ++m) {
     if (k[n - m] != -k[m]) return false;
-    if (std::fabs(k[M - m] - (-k[M] - k[m])) > 64.0 * 2.220446049250313e-16 * scale) return false;
+    if (std::fabs(k[m] - m * k[1]) > tol) return false;
   }
+  return true;
+}
+
+// v_m = v0 + m dvv to rounding (the collision kernels form v_m that way)
+bool uniform_grid(const double* v, int n, double* v0, double* dvv) {
+  *v0 = v[0];
+  *dvv = n > 1 ? (v[n - 1] - v[0]) / (n - 1) : 0.0;
+  double vmax = 0.0;
+  for (int m = 0; m < n; ++m) vmax = std::fmax(vmax, std::fabs(v[m]));
+  for (int m = 0; m < n; ++m)
+    if (std::fabs(*v0 + m * *dvv - v[m]) > 16.0 * 2.220446049250313e-16 * vmax) return false;
   return true;
 }
 
@@ -244,17 +270,29 @@ bool fftfreq_like(const double* k, int n) {
 struct vpfp_plan_s {
   int nx = 0, nv = 0, prec = VPFP_F64, op = VPFP_OP_LB;
   double dv = 0, nu = 0, dt = 0;
+  double dk_x = 0, dk_v = 0, v0 = 0, dvv = 0;
   bool nx_pow2 = false, nv_pow2 = false;
   int C = 0, ntiles = 0;
   double *kx = nullptr, *okx = nullptr, *v = nullptr, *kv = nullptr;
-  void* tw_v = nullptr;          // cx<T>[nv]
-  void* tw_x = nullptr;          // cx<T>[nx]
-  cx<double>* tw_xd = nullptr;   // cx<double>[nx]
+  void* tw_v = nullptr;          // cx<T> [lo | hi] tables for nv
+  void* tw_x = nullptr;          // cx<T> [lo | hi] tables for nx
+  cx<double>* tw_xs = nullptr;   // cx<double> [lo | hi] tables for nx (field)
+  cx<double>* tw_xd = nullptr;   // cx<double>[nx] full table (direct-DFT field)
   double* dens = nullptr;        // [max(ntiles,1)][nx]
   double* ering = nullptr;       // [2][nx]
+  double* rho_g = nullptr;       // [nx] field reduction scratch
+  unsigned int* counter = nullptr;
   double* series = nullptr;      // [2]
+  // per-kernel event timing (vpfp_plan_set_profiling)
+  bool profiling = false;
+  double kernel_ms[4] = {0, 0, 0, 0};  // row, drift, field, other
+  long long kernel_n[4] = {0, 0, 0, 0};
+  std::vector<hipEvent_t> ev;          // pool, 2 per timed launch
+  std::vector<int> ev_kind;
+  size_t ev_used = 0;
   std::vector<void*> allocs;
   ~vpfp_plan_s() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
   }
 };
@@ -288,19 +326,34 @@ std::vector<cx<T>> twiddles(int N) {
   return w;
 }
 
+// [lo | hi] tables: lo[j] = W_N^j (j < 64), hi[h] = W_N^{64 h} (h < max(1, N/64))
+template <class T>
+std::vector<cx<T>> split_twiddles(int N) {
+  const int nhi = N / 64 > 1 ? N / 64 : 1;
+  std::vector<cx<T>> w(64 + nhi);
+  const long double two_pi = 6.283185307179586476925286766559L;
+  for (int j = 0; j < 64 + nhi; ++j) {
+    const long double n = j < 64 ? (long double)j : (long double)(64 * (j - 64));
+    const long double a = two_pi * n / (long double)N;
+    w[j] = vp::mk<T>((T)(double)cosl(a), (T)(double)(-sinl(a)));
+  }
+  return w;
+}
+
+template <class T, class P>
+int put(vpfp_plan_s* p, P** dst, const std::vector<T>& src) {
+  if (int rc = dalloc(p, dst, sizeof(T) * src.size())) return rc;
+  return upload(*dst, src.data(), sizeof(T) * src.size());
+}
+
 template <class T>
 int make_tables(vpfp_plan_s* p) {
-  auto wv = twiddles<T>(p->nv);
-  auto wx = twiddles<T>(p->nx);
-  auto wxd = twiddles<double>(p->nx);
   cx<T>* dv_ = nullptr;
   cx<T>* dx_ = nullptr;
-  if (int rc = dalloc(p, &dv_, sizeof(cx<T>) * p->nv)) return rc;
-  if (int rc = dalloc(p, &dx_, sizeof(cx<T>) * p->nx)) return rc;
-  if (int rc = dalloc(p, &p->tw_xd, sizeof(cx<double>) * p->nx)) return rc;
-  if (int rc = upload(dv_, wv.data(), sizeof(cx<T>) * p->nv)) return rc;
-  if (int rc = upload(dx_, wx.data(), sizeof(cx<T>) * p->nx)) return rc;
-  if (int rc = upload(p->tw_xd, wxd.data(), sizeof(cx<double>) * p->nx)) return rc;
+  if (int rc = put(p, &dv_, split_twiddles<T>(p->nv))) return rc;
+  if (int rc = put(p, &dx_, split_twiddles<T>(p->nx))) return rc;
+  if (int rc = put(p, &p->tw_xs, split_twiddles<double>(p->nx))) return rc;
+  if (int rc = put(p, &p->tw_xd, twiddles<double>(p->nx))) return rc;
   p->tw_v = dv_;
   p->tw_x = dx_;
   return VPFP_OK;
@@ -315,6 +368,48 @@ int drift_tile(int nx, int nv) {
 
 hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+enum KernelKind { KK_ROW = 0, KK_DRIFT = 1, KK_FIELD = 2, KK_OTHER = 3 };
+
+// In profiling mode every launch of the fused step is bracketed by two
+// events; vpfp_step_n reads them back once its stream has drained.
+int prof_mark(vpfp_plan_t p, int kind, hipStream_t s) {
+  if (!p->profiling) return VPFP_OK;
+  if (p->ev_used + 1 >= p->ev.size()) {
+    const size_t grow = p->ev.size() + 256;
+    while (p->ev.size() < grow) {
+      hipEvent_t e;
+      VPFP_HIP(hipEventCreate(&e));
+      p->ev.push_back(e);
+    }
+    p->ev_kind.resize(p->ev.size() / 2 + 1);
+  }
+  if ((p->ev_used & 1) == 0) p->ev_kind[p->ev_used / 2] = kind;
+  VPFP_HIP(hipEventRecord(p->ev[p->ev_used], s));
+  ++p->ev_used;
+  return VPFP_OK;
+}
+
+int prof_collect(vpfp_plan_t p) {
+  if (!p->profiling || p->ev_used == 0) return VPFP_OK;
+  VPFP_HIP(hipEventSynchronize(p->ev[p->ev_used - 1]));
+  for (size_t i = 0; i + 1 < p->ev_used; i += 2) {
+    float ms = 0.f;
+    VPFP_HIP(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
+    const int k = p->ev_kind[i / 2];
+    p->kernel_ms[k] += ms;
+    p->kernel_n[k] += 1;
+  }
+  p->ev_used = 0;
+  return VPFP_OK;
+}
+
+#define PROF(kind, call)                                  \
+  do {                                                    \
+    if (int rc_ = prof_mark(p, kind, s)) return rc_;      \
+    if (int rc_ = (call)) return rc_;                     \
+    if (int rc_ = prof_mark(p, kind, s)) return rc_;      \
+  } while (0)
+
 template <class T>
 vp::RowArgs<T> row_args(vpfp_plan_t p, const void* fin, void* fout) {
   vp::RowArgs<T> a;
@@ -323,8 +418,9 @@ vp::RowArgs<T> row_args(vpfp_plan_t p, const void* fin, void* fout) {
   a.f_out = reinterpret_cast<T*>(fout);
   a.nx = p->nx;
   a.nv = p->nv;
-  a.kv = p->kv;
-  a.v = p->v;
+  a.dk = p->dk_v;
+  a.v0 = p->v0;
+  a.dvv = p->dvv;
   a.dv = p->dv;
   a.tw = reinterpret_cast<const cx<T>*>(p->tw_v);
   a.nu_dt = p->nu * p->dt;
@@ -340,7 +436,7 @@ vp::DriftArgs<T> drift_args(vpfp_plan_t p, const void* fin, void* fout, double d
   a.nx = p->nx;
   a.nv = p->nv;
   a.dt = dt;
-  a.kx = p->kx;
+  a.dk = p->dk_x;
   a.v = p->v;
   a.tw = reinterpret_cast<const cx<T>*>(p->tw_x);
   a.dens = dens;
@@ -348,14 +444,16 @@ vp::DriftArgs<T> drift_args(vpfp_plan_t p, const void* fin, void* fout, double d
 }
 
 vp::FieldArgs field_args(vpfp_plan_t p, const double* part, int ntiles, double scale, const double* driver,
-                         double* e_out, double* e_out2, double* series) {
+                         double* e_out, double* e_out2, double* series, bool pow2) {
   vp::FieldArgs a;
   a.part = part;
   a.ntiles = ntiles;
+  a.rho_g = p->rho_g;
+  a.counter = p->counter;
   a.nx = p->nx;
   a.scale = scale;
   a.okx = p->okx;
-  a.tw = p->tw_xd;
+  a.tw = pow2 ? p->tw_xs : p->tw_xd;
   a.driver = driver;
   a.e_out = e_out;
   a.e_out2 = e_out2;
@@ -443,14 +541,14 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
   ra.e = e;
   ra.dt_a = half;
   ra.flags = vp::RK_KICK_A;
-  if (int rc = row_dispatch<T>(ra, s)) return rc;
+  if (n > 0) PROF(KK_ROW, row_dispatch<T>(ra, s));
   const double* e_cur = e;
   for (int i = 0; i < n; ++i) {
-    if (int rc = do_vdfdx<T>(p, f, f, p->dt, p->dens, s)) return rc;
+    PROF(KK_DRIFT, (do_vdfdx<T>(p, f, f, p->dt, p->dens, s)));
     double* ei = e_hist ? e_hist + (size_t)i * p->nx : p->ering + (size_t)(i & 1) * p->nx;
     auto fa = field_args(p, p->dens, p->ntiles, p->dv, driver ? driver + (size_t)i * p->nx : nullptr, ei, nullptr,
-                         series ? series + 2 * (size_t)i : nullptr);
-    if (int rc = field_dispatch(fa, true, s)) return rc;
+                         series ? series + 2 * (size_t)i : nullptr, true);
+    PROF(KK_FIELD, field_dispatch(fa, true, s));
     e_cur = ei;
     auto rb = row_args<T>(p, f, f);
     rb.e = e_cur;
@@ -459,10 +557,10 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
     rb.moments = moments ? moments + (size_t)i * vp::kMoments * p->nx : nullptr;
     rb.flags = vp::RK_KICK_A | (coll ? vp::RK_COLLIDE : 0) | (moments ? vp::RK_MOMENTS : 0) |
                (i + 1 < n ? vp::RK_KICK_B : 0);
-    if (int rc = row_dispatch<T>(rb, s)) return rc;
+    PROF(KK_ROW, row_dispatch<T>(rb, s));
   }
   if (n > 0) VPFP_HIP(hipMemcpyAsync(e, e_cur, sizeof(double) * p->nx, hipMemcpyDeviceToDevice, s));
-  return VPFP_OK;
+  return prof_collect(p);
 }
 
 }  // namespace
@@ -486,6 +584,8 @@ int vpfp_plan_create(const vpfp_grid_desc* d, vpfp_plan_t* out) {
   if (!(d->dv > 0.0)) return fail(VPFP_EBADARG, "dv must be positive");
   if (!fftfreq_like(d->kx, d->nx)) return fail(VPFP_EUNSUPPORTED, "kx is not an fftfreq wavenumber table");
   if (!fftfreq_like(d->kv, d->nv)) return fail(VPFP_EUNSUPPORTED, "kv is not an fftfreq wavenumber table");
+  double v0 = 0.0, dvv = 0.0;
+  if (!uniform_grid(d->v, d->nv, &v0, &dvv)) return fail(VPFP_EUNSUPPORTED, "v is not a uniform grid");
 
   auto* p = new vpfp_plan_s();
   p->nx = d->nx;
@@ -495,10 +595,14 @@ int vpfp_plan_create(const vpfp_grid_desc* d, vpfp_plan_t* out) {
   p->dv = d->dv;
   p->nu = d->nu;
   p->dt = d->dt;
+  p->dk_x = d->nx > 1 ? d->kx[1] : 0.0;
+  p->dk_v = d->nv > 1 ? d->kv[1] : 0.0;
+  p->v0 = v0;
+  p->dvv = dvv;
   p->nx_pow2 = is_pow2(d->nx);
   p->nv_pow2 = is_pow2(d->nv);
   p->C = (p->nx_pow2 && p->nv_pow2 && p->nx >= 4) ? drift_tile(p->nx, p->nv) : 0;
-  p->ntiles = p->C ? p->nv / p->C : 0;
+  p->ntiles = p->C ? drift_grid(p->nv / p->C) : 0;
   int rc = VPFP_OK;
   const size_t nxb = sizeof(double) * d->nx, nvb = sizeof(double) * d->nv;
   if (!rc) rc = dalloc(p, &p->kx, nxb);
@@ -513,6 +617,12 @@ int vpfp_plan_create(const vpfp_grid_desc* d, vpfp_plan_t* out) {
   if (!rc) rc = dalloc(p, &p->dens, nxb * (size_t)(p->ntiles > 1 ? p->ntiles : 1));
   if (!rc) rc = dalloc(p, &p->ering, 2 * nxb);
   if (!rc) rc = dalloc(p, &p->series, 2 * sizeof(double));
+  if (!rc) rc = dalloc(p, &p->rho_g, nxb);
+  if (!rc) rc = dalloc(p, &p->counter, 64);
+  if (!rc) {
+    hipError_t e = hipMemset(p->counter, 0, 64);
+    if (e != hipSuccess) rc = fail(VPFP_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  }
   if (rc) {
     delete p;
     return rc;
@@ -555,8 +665,9 @@ int vpfp_solve_for_field(vpfp_plan_t p, const double* n, double* e_out, void* st
   if (int rc = check_plan(p)) return rc;
   if (!n || !e_out) return fail(VPFP_EBADARG, "null array");
   if (p->nx < 2) return fail(VPFP_EUNSUPPORTED, "nx must be at least 2");
-  auto fa = field_args(p, n, 1, 1.0, nullptr, e_out, nullptr, nullptr);
-  return field_dispatch(fa, p->nx_pow2 && p->nx >= 4 && p->nx <= 4096, S(stream));
+  const bool pow2 = p->nx_pow2 && p->nx >= 4 && p->nx <= 4096;
+  auto fa = field_args(p, n, 1, 1.0, nullptr, e_out, nullptr, nullptr, pow2);
+  return field_dispatch(fa, pow2, S(stream));
 }
 
 int vpfp_field(vpfp_plan_t p, const void* f, const double* driver, double* e_out, void* stream) {
@@ -565,8 +676,9 @@ int vpfp_field(vpfp_plan_t p, const void* f, const double* driver, double* e_out
   int rc = p->prec == VPFP_F64 ? do_density<double>(p, f, p->dv, p->dens, S(stream))
                                : do_density<float>(p, f, p->dv, p->dens, S(stream));
   if (rc) return rc;
-  auto fa = field_args(p, p->dens, 1, 1.0, driver, e_out, nullptr, nullptr);
-  return field_dispatch(fa, p->nx_pow2 && p->nx >= 4 && p->nx <= 4096, S(stream));
+  const bool pow2 = p->nx_pow2 && p->nx >= 4 && p->nx <= 4096;
+  auto fa = field_args(p, p->dens, 1, 1.0, driver, e_out, nullptr, nullptr, pow2);
+  return field_dispatch(fa, pow2, S(stream));
 }
 
 int vpfp_collision_diagonals(vpfp_plan_t p, const void* f, double* a, double* b, double* c, void* stream) {
@@ -596,6 +708,27 @@ int vpfp_collide(vpfp_plan_t p, const void* fin, void* fout, void* stream) {
   }
   if (int rc = need_spectral_v(p)) return rc;
   return p->prec == VPFP_F64 ? do_collide<double>(p, fin, fout, S(stream)) : do_collide<float>(p, fin, fout, S(stream));
+}
+
+int vpfp_plan_set_profiling(vpfp_plan_t p, int on) {
+  if (int rc = check_plan(p)) return rc;
+  p->profiling = on != 0;
+  for (int k = 0; k < 4; ++k) {
+    p->kernel_ms[k] = 0.0;
+    p->kernel_n[k] = 0;
+  }
+  p->ev_used = 0;
+  return VPFP_OK;
+}
+
+int vpfp_plan_kernel_times(vpfp_plan_t p, double* ms, long long* counts) {
+  if (int rc = check_plan(p)) return rc;
+  if (!ms || !counts) return fail(VPFP_EBADARG, "null array");
+  for (int k = 0; k < 4; ++k) {
+    ms[k] = p->kernel_ms[k];
+    counts[k] = p->kernel_n[k];
+  }
+  return VPFP_OK;
 }
 
 int vpfp_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, double* e_hist, double* moments,

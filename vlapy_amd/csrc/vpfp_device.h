@@ -1,10 +1,13 @@
 // Device building blocks for the VPFP kernels (gfx950, wave64).
 //
 // Everything here is written against a small block context `Ctx` that
-// provides `tid` (thread index in the workgroup), `sync()` (workgroup
-// barrier) and `shfl_xor(v, mask, width)` (lane exchange inside a
-// wavefront).  The HIP kernels in vpfp.hip instantiate it with the hardware
-// primitives.
+// provides `tid`, `nthr` (thread index and count in the workgroup), `bid`,
+// `nblk` (workgroup index and count), `sync()` (workgroup barrier),
+// `shfl_xor(v, mask, width)` (lane exchange inside a wavefront) and, for the
+// cross-workgroup hand-off of the field kernel, `fence()` (device-scope
+// fence), `arrive(counter)` (device-scope atomic increment, returns the old
+// value), `load_coherent(p)` and `reset(counter)`.  The HIP kernels in
+// vpfp.hip instantiate it with the hardware primitives.
 //
 // Contents
 //   * cx<T>        : complex value in registers / LDS (T = double or float)
@@ -155,6 +158,33 @@ VP_HD void dft(cx<T>* x) {
 }
 
 // ---------------------------------------------------------------------------
+// Twiddles W_N^i = exp(-2 pi i i / N) from two small tables staged in LDS:
+// W_N^i = hi[i >> 6] * lo[i & 63], hi[h] = W_N^{64 h}, lo[j] = W_N^j.
+// The global layout is [lo: 64 entries | hi: max(1, N/64) entries].
+// ---------------------------------------------------------------------------
+template <int N>
+struct TwSize {
+  static constexpr int hi = N / 64 > 1 ? N / 64 : 1;
+  static constexpr int total = 64 + hi;
+};
+
+template <class T>
+struct Tw {
+  const cx<T>* t;  // LDS copy, [lo | hi]
+  VP_HD cx<T> operator()(int i) const {
+    const cx<T> lo = t[i & 63];
+    if (i < 64) return lo;
+    return mul(t[64 + (i >> 6)], lo);
+  }
+};
+
+// all threads copy the table to LDS (no barrier here)
+template <class T, int N, class Ctx>
+VP_HD void tw_stage(Ctx& c, cx<T>* dst, const cx<T>* src) {
+  for (int i = c.tid; i < TwSize<N>::total; i += c.nthr) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
 // M-point complex FFT held by TPT threads (Stockham autosort, radix <= 16)
 //
 // Register layout: thread t holds element t + i*TPT in v[i], i < Q = M/TPT.
@@ -164,11 +194,11 @@ VP_HD void dft(cx<T>* x) {
 // position (j - j mod NS)*R + j mod NS + r*NS.  After the last pass the data
 // are in natural order, both in v and in `buf`.
 //
-// tw is a table of exp(-2 pi i n / (M*twmul)), n < M*twmul.
+// tw(n) = exp(-2 pi i n / (2M)); the M-point twiddle W_M^n is tw(2n).
 // ---------------------------------------------------------------------------
 // `act` false: the thread only takes part in the barriers (no LDS writes).
 template <class T, int M, int TPT, int R, int NS, bool INV, class Ctx>
-VP_HD void st_pass(Ctx& c, cx<T>* v, cx<T>* buf, const cx<T>* tw, int twmul, int t, bool act) {
+VP_HD void st_pass(Ctx& c, cx<T>* v, cx<T>* buf, const Tw<T>& tw, int t, bool act) {
   constexpr int Q = M / TPT;
   constexpr int NB = Q / R;
   static_assert(NB * R == Q, "radix must divide the per-thread count");
@@ -182,7 +212,7 @@ VP_HD void st_pass(Ctx& c, cx<T>* v, cx<T>* buf, const cx<T>* tw, int twmul, int
     if constexpr (NS > 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) {
-        const cx<T> w = tw[(r * k * (M / (NS * R))) * twmul];
+        const cx<T> w = tw(2 * (r * k * (M / (NS * R))));
         u[r] = INV ? mulc(u[r], w) : mul(u[r], w);
       }
     }
@@ -200,13 +230,13 @@ VP_HD void st_pass(Ctx& c, cx<T>* v, cx<T>* buf, const cx<T>* tw, int twmul, int
 }
 
 template <class T, int M, int TPT, bool INV, int NS = 1, class Ctx>
-VP_HD void fft(Ctx& c, cx<T>* v, cx<T>* buf, const cx<T>* tw, int twmul, int t, bool act = true) {
+VP_HD void fft(Ctx& c, cx<T>* v, cx<T>* buf, const Tw<T>& tw, int t, bool act = true) {
   constexpr int Q = M / TPT;
   static_assert(Q >= 2 && Q * TPT == M, "each thread must hold at least two values");
   constexpr int RMAX = Q < 16 ? Q : 16;
   constexpr int R = (M / NS) < RMAX ? (M / NS) : RMAX;
-  st_pass<T, M, TPT, R, NS, INV>(c, v, buf, tw, twmul, t, act);
-  if constexpr (NS * R < M) fft<T, M, TPT, INV, NS * R>(c, v, buf, tw, twmul, t, act);
+  st_pass<T, M, TPT, R, NS, INV>(c, v, buf, tw, t, act);
+  if constexpr (NS * R < M) fft<T, M, TPT, INV, NS * R>(c, v, buf, tw, t, act);
 }
 
 // ---------------------------------------------------------------------------
@@ -217,35 +247,39 @@ VP_HD void fft(Ctx& c, cx<T>* v, cx<T>* buf, const cx<T>* tw, int twmul, int t, 
 // of the multiplier, which is what taking Re() of the complex inverse leaves).
 // ---------------------------------------------------------------------------
 
-// Phase shift exp(-i (kvec[k]*dt)*s) of the exponential pushers.  kvec is an
-// fftfreq wavenumber table (checked on the host: kvec[N-m] == -kvec[m] and
-// linear in m), so P_{M-k} = exp(-i theta_M) * conj(P_k) with
-// theta_M = (-kvec[M]*dt)*s the positive Nyquist phase.
+// Phase shift exp(-i theta_k) of the exponential pushers, theta_k = k phi
+// with phi = (dk * dt) * s: dk is the first wavenumber of an fftfreq table
+// (checked on the host to be linear in k to rounding), s the advection speed
+// (e(x) for the v-kick, v for the x-drift).  Bins are visited in the order
+// k = t, t + TPT, t + 2 TPT, ... so P_k is a running product; the partner
+// phase is P_{M-k} = exp(-i theta_M) conj(P_k).
 template <class T>
 struct PhaseMul {
-  const double* kvec;
-  double dt, s;
-  cx<T> pm;  // exp(-i theta_M)
-  VP_HD PhaseMul(const double* kvec_, double dt_, double s_, double kvec_nyq) : kvec(kvec_), dt(dt_), s(s_) {
-    cis_neg((-kvec_nyq * dt) * s, pm);
+  cx<T> cur, step, pm;
+  VP_HD PhaseMul(double phi, int M, int t, int TPT) {
+    cis_neg((double)M * phi, pm);
+    cis_neg((double)t * phi, cur);
+    cis_neg((double)TPT * phi, step);
   }
-  VP_HD void pair(int k, cx<T>& pk, cx<T>& pmk) const {
+  VP_HD void pair(int k, cx<T>& pk, cx<T>& pmk) {
     if (k == 0) {
       pk = mk<T>(T(1), T(0));
       pmk = mk<T>(pm.re, T(0));
-      return;
+    } else {
+      pk = cur;
+      pmk = mulc(pm, pk);
     }
-    cis_neg((kvec[k] * dt) * s, pk);
-    pmk = mulc(pm, pk);
+    cur = mul(cur, step);
   }
 };
 
-// Poisson multiplier i * one_over_kx (zero real part at DC and Nyquist).
+// Poisson multiplier i * one_over_kx (zero real part at DC and Nyquist);
+// okx is an LDS copy of the one_over_kx table.
 template <class T>
 struct PoissonMul {
   const double* okx;
   int M;
-  VP_HD void pair(int k, cx<T>& pk, cx<T>& pmk) const {
+  VP_HD void pair(int k, cx<T>& pk, cx<T>& pmk) {
     if (k == 0) {
       pk = mk<T>(T(0), T(0));
       pmk = mk<T>(T(0), T(0));
@@ -262,9 +296,9 @@ struct PoissonMul {
 // F_{k+M} = E_k - W O_k.  After F'_k = P_k F_k, F'_{k+M} = conj(P_{M-k}) F_{k+M}
 // the repacked spectrum Z'_k = (F'_k + F'_{k+M} + i (F'_k - F'_{k+M}) conj W)/N
 // inverse-transforms (unnormalised M-point) to f'_{2j} + i f'_{2j+1}.
-// v[i] holds Z_k and pz[i] holds Z_{M-k} for k = t + i*TPT; twN[k] = W^k.
+// v[i] holds Z_k and pz[i] holds Z_{M-k} for k = t + i*TPT; tw(k) = W^k.
 template <class T, int M, int TPT, class Mul>
-VP_HD void couple(cx<T>* v, const cx<T>* pz, const cx<T>* twN, int t, const Mul& mult) {
+VP_HD void couple(cx<T>* v, const cx<T>* pz, const Tw<T>& tw, int t, Mul& mult) {
   constexpr int Q = M / TPT;
   const T invN = T(1) / T(2 * M);
   const T h = T(0.5);
@@ -274,7 +308,7 @@ VP_HD void couple(cx<T>* v, const cx<T>* pz, const cx<T>* twN, int t, const Mul&
     const cx<T> a = v[i], p = pz[i];
     const cx<T> E = mk<T>((a.re + p.re) * h, (a.im - p.im) * h);
     const cx<T> O = mk<T>((a.im + p.im) * h, (p.re - a.re) * h);
-    const cx<T> W = twN[k];
+    const cx<T> W = tw(k);
     const cx<T> WO = mul(W, O);
     const cx<T> Fk = add(E, WO), Fkm = sub(E, WO);
     cx<T> pk, pmk;

@@ -32,16 +32,15 @@ VP_HD double trapw(int m, int N) { return (m == 0 || m == N - 1) ? 0.5 : 1.0; }
 // spectral multiply along a length-2M real sequence held as M complex values
 // ---------------------------------------------------------------------------
 template <class T, int M, int TPT, class Mul, class Ctx>
-VP_HD void spectral_apply(Ctx& c, cx<T>* v, cx<T>* buf, const cx<T>* tw, int t, const Mul& mult,
-                          bool act = true) {
+VP_HD void spectral_apply(Ctx& c, cx<T>* v, cx<T>* buf, const Tw<T>& tw, int t, Mul mult, bool act = true) {
   constexpr int Q = M / TPT;
-  fft<T, M, TPT, false>(c, v, buf, tw, 2, t, act);
+  fft<T, M, TPT, false>(c, v, buf, tw, t, act);
   cx<T> pz[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) pz[i] = buf[pidx((M - (t + i * TPT)) & (M - 1))];
   c.sync();
   couple<T, M, TPT>(v, pz, tw, t, mult);
-  fft<T, M, TPT, true>(c, v, buf, tw, 2, t, act);
+  fft<T, M, TPT, true>(c, v, buf, tw, t, act);
 }
 
 // ---------------------------------------------------------------------------
@@ -50,12 +49,13 @@ VP_HD void spectral_apply(Ctx& c, cx<T>* v, cx<T>* buf, const cx<T>* tw, int t, 
 //   u(m) = nu dt (-T/dv^2 - (v_{m+1} - vbar)/(2 dv)),  m <= N-2
 //   b    = 1 + nu dt (2 T / dv^2)
 // ---------------------------------------------------------------------------
+// v_m = v0 + m dvv (the host checks that the grid is uniform to rounding)
 struct CollCoef {
-  const double* v;
+  double v0, dvv;
   int N;
   double ndt, A0, H, vbar, bd;
-  VP_HD double l(int m) const { return m == 0 ? 0.0 : ndt * (A0 + (v[m - 1] - vbar) * H); }
-  VP_HD double u(int m) const { return m == N - 1 ? 0.0 : ndt * (A0 - (v[m + 1] - vbar) * H); }
+  VP_HD double l(int m) const { return m == 0 ? 0.0 : ndt * (A0 + ((v0 + (m - 1) * dvv) - vbar) * H); }
+  VP_HD double u(int m) const { return m == N - 1 ? 0.0 : ndt * (A0 - ((v0 + (m + 1) * dvv) - vbar) * H); }
   VP_HD double b(int) const { return bd; }
 };
 
@@ -72,21 +72,21 @@ struct ArrCoef {
 
 // drift velocity and temperature of one row (un-normalised, trapz), chunk layout
 template <int TPT, int L, class Ctx>
-VP_HD void coll_moments(Ctx& c, const double* d, const double* v, int N, double dv, bool dg, int t,
+VP_HD void coll_moments(Ctx& c, const double* d, double v0, double dvv, int N, double dv, bool dg, int t,
                         double* rs, double& vbar, double& temp) {
   const int m0 = t * L;
   double s[1] = {0.0};
   vbar = 0.0;
   if (dg) {
 #pragma unroll
-    for (int i = 0; i < L; ++i) s[0] += trapw(m0 + i, N) * d[i] * v[m0 + i];
+    for (int i = 0; i < L; ++i) s[0] += trapw(m0 + i, N) * d[i] * (v0 + (m0 + i) * dvv);
     row_sum<TPT, 1>(c, s, rs);
     vbar = dv * s[0];
     s[0] = 0.0;
   }
 #pragma unroll
   for (int i = 0; i < L; ++i) {
-    const double w = v[m0 + i] - vbar;
+    const double w = (v0 + (m0 + i) * dvv) - vbar;
     s[0] += trapw(m0 + i, N) * d[i] * (w * w);
   }
   row_sum<TPT, 1>(c, s, rs);
@@ -103,10 +103,10 @@ struct RowArgs {
   int nx, nv;
   const double* e;  // [nx] field for the kicks
   double dt_a, dt_b;
-  const double* kv;  // [nv]
-  const double* v;   // [nv]
+  double dk;         // kv[1] of the fftfreq table
+  double v0, dvv;    // v_m = v0 + m dvv
   double dv;
-  const cx<T>* tw;  // exp(-2 pi i n / nv), n < nv
+  const cx<T>* tw;   // twiddle tables of exp(-2 pi i n / nv), [lo | hi]
   double nu_dt;
   int op_dg;
   double* moments;  // [kMoments][nx]
@@ -117,13 +117,14 @@ template <class T, int M, int Q>
 struct RowCfg {
   static constexpr int TPT = M / Q;
   static constexpr int L = 2 * Q;
+  static constexpr int tw_bytes = (TwSize<2 * M>::total * (int)sizeof(cx<T>) + 15) & ~15;
   static constexpr int fft_bytes = Padded<M>::size * (int)sizeof(cx<T>);
   static constexpr int red_bytes = 8 * TPT * 8;
   static constexpr int region = ((fft_bytes > red_bytes ? fft_bytes : red_bytes) + 15) & ~15;
   static constexpr int wg = TPT >= 64 ? TPT : 64;
   static constexpr int rows = wg / TPT;
   static constexpr int sum_bytes = (TPT > 64 ? TPT / 64 : 1) * kMoments * 8;
-  static constexpr int lds = rows * region + sum_bytes;
+  static constexpr int lds = tw_bytes + rows * region + sum_bytes;
 };
 
 template <class T, int M, int Q, class Ctx>
@@ -136,18 +137,23 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
   const int row = c.bid * Cfg::rows + rl;
   const bool valid = row < a.nx;
   const int rowc = valid ? row : a.nx - 1;
-  cx<T>* buf = reinterpret_cast<cx<T>*>(lds + rl * Cfg::region);
-  double* red = reinterpret_cast<double*>(lds + rl * Cfg::region);
-  double* rs = reinterpret_cast<double*>(lds + Cfg::rows * Cfg::region);
+  cx<T>* twl = reinterpret_cast<cx<T>*>(lds);
+  unsigned char* base = lds + Cfg::tw_bytes;
+  cx<T>* buf = reinterpret_cast<cx<T>*>(base + rl * Cfg::region);
+  double* red = reinterpret_cast<double*>(base + rl * Cfg::region);
+  double* rs = reinterpret_cast<double*>(base + Cfg::rows * Cfg::region);
 
   const cx<T>* src = reinterpret_cast<const cx<T>*>(a.f_in + (size_t)rowc * a.nv);
   cx<T> v[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) v[i] = src[t + i * TPT];
-
   const double e = (a.flags & (RK_KICK_A | RK_KICK_B)) ? a.e[rowc] : 0.0;
+  tw_stage<T, 2 * M>(c, twl, a.tw);
+  c.sync();
+  const Tw<T> tw{twl};
+
   if (a.flags & RK_KICK_A)
-    spectral_apply<T, M, TPT>(c, v, buf, a.tw, t, PhaseMul<T>(a.kv, a.dt_a, e, a.kv[M]));
+    spectral_apply<T, M, TPT>(c, v, buf, tw, t, PhaseMul<T>((a.dk * a.dt_a) * e, M, t, TPT));
 
   if (a.flags & (RK_COLLIDE | RK_MOMENTS)) {
     // natural order -> contiguous chunk of L samples per thread
@@ -166,10 +172,11 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
     const int m0 = t * L;
     if ((a.flags & RK_COLLIDE) && a.nu_dt > 0.0) {
       double vbar, temp;
-      coll_moments<TPT, L>(c, d, a.v, N, a.dv, a.op_dg != 0, t, rs, vbar, temp);
+      coll_moments<TPT, L>(c, d, a.v0, a.dvv, N, a.dv, a.op_dg != 0, t, rs, vbar, temp);
       const double dv2 = a.dv * a.dv;
       CollCoef cf;
-      cf.v = a.v;
+      cf.v0 = a.v0;
+      cf.dvv = a.dvv;
       cf.N = N;
       cf.ndt = a.nu_dt;
       cf.A0 = -temp / dv2;
@@ -186,7 +193,7 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
       for (int i = 0; i < L; ++i) {
         const int m = m0 + i;
         const double wf = trapw(m, N) * d[i];
-        const double vm = a.v[m];
+        const double vm = a.v0 + m * a.dvv;
         double p = wf;
         s[0] += p;
         p *= vm;
@@ -218,7 +225,7 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
   }
 
   if (a.flags & RK_KICK_B)
-    spectral_apply<T, M, TPT>(c, v, buf, a.tw, t, PhaseMul<T>(a.kv, a.dt_b, e, a.kv[M]));
+    spectral_apply<T, M, TPT>(c, v, buf, tw, t, PhaseMul<T>((a.dk * a.dt_b) * e, M, t, TPT));
 
   if (valid) {
     cx<T>* dst = reinterpret_cast<cx<T>*>(a.f_out + (size_t)row * a.nv);
@@ -236,10 +243,10 @@ struct DriftArgs {
   T* f_out;
   int nx, nv;
   double dt;
-  const double* kx;  // [nx]
+  double dk;         // kx[1] of the fftfreq table
   const double* v;   // [nv]
-  const cx<T>* tw;   // exp(-2 pi i n / nx), n < nx
-  double* dens;      // [nv / C][nx] partial sums (unscaled), or null
+  const cx<T>* tw;   // twiddle tables of exp(-2 pi i n / nx), [lo | hi]
+  double* dens;      // [gridDim][nx] partial sums (unscaled), or null
 };
 
 template <class T, int M, int Q, int C>
@@ -247,7 +254,8 @@ struct DriftCfg {
   static constexpr int TPT = M / Q;
   static constexpr int wg = TPT * C;
   static constexpr int stride = Padded<M>::size;
-  static constexpr int lds = C * stride * (int)sizeof(cx<T>);
+  static constexpr int tw_bytes = (TwSize<2 * M>::total * (int)sizeof(cx<T>) + 15) & ~15;
+  static constexpr int lds = tw_bytes + C * stride * (int)sizeof(cx<T>);
 };
 
 template <class T, int M, int Q, int C, class Ctx>
@@ -256,39 +264,54 @@ VP_HD void drift_body(Ctx& c, unsigned char* lds, const DriftArgs<T>& a) {
   constexpr int TPT = Cfg::TPT;
   const int cc = c.tid % C;
   const int t = c.tid / C;
-  const int col = c.bid * C + cc;
-  cx<T>* buf = reinterpret_cast<cx<T>*>(lds) + cc * Cfg::stride;
+  cx<T>* twl = reinterpret_cast<cx<T>*>(lds);
+  cx<T>* buf = reinterpret_cast<cx<T>*>(lds + Cfg::tw_bytes) + cc * Cfg::stride;
   const size_t nv = (size_t)a.nv;
-  const T* fi = a.f_in + col;
-  cx<T> v[Q];
+  tw_stage<T, 2 * M>(c, twl, a.tw);
+  const Tw<T> tw{twl};
+  double acc[2 * Q];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const int j = t + i * TPT;
-    v[i] = mk<T>(fi[(size_t)(2 * j) * nv], fi[(size_t)(2 * j + 1) * nv]);
-  }
-  spectral_apply<T, M, TPT>(c, v, buf, a.tw, t, PhaseMul<T>(a.kx, a.dt, a.v[col], a.kx[M]));
-  T* fo = a.f_out + col;
+  for (int i = 0; i < 2 * Q; ++i) acc[i] = 0.0;
+  const int ntiles = a.nv / C;
+  // the grid strides over the column tiles; workgroup b accumulates the
+  // density of every tile it advects into one partial row dens[b][x]
+  for (int tile = c.bid; tile < ntiles; tile += c.nblk) {
+    const int col = tile * C + cc;
+    const T* fi = a.f_in + col;
+    cx<T> v[Q];
 #pragma unroll
-  for (int i = 0; i < Q; ++i) {
-    const int j = t + i * TPT;
-    fo[(size_t)(2 * j) * nv] = v[i].re;
-    fo[(size_t)(2 * j + 1) * nv] = v[i].im;
+    for (int i = 0; i < Q; ++i) {
+      const int j = t + i * TPT;
+      v[i] = mk<T>(fi[(size_t)(2 * j) * nv], fi[(size_t)(2 * j + 1) * nv]);
+    }
+    const double vc = a.v[col];
+    c.sync();  // twiddle tables staged (first tile) / buffer free (later tiles)
+    spectral_apply<T, M, TPT>(c, v, buf, tw, t, PhaseMul<T>((a.dk * a.dt) * vc, M, t, TPT));
+    T* fo = a.f_out + col;
+    const double w = trapw(col, a.nv);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int j = t + i * TPT;
+      fo[(size_t)(2 * j) * nv] = v[i].re;
+      fo[(size_t)(2 * j + 1) * nv] = v[i].im;
+      acc[2 * i] += w * (double)v[i].re;
+      acc[2 * i + 1] += w * (double)v[i].im;
+    }
   }
   if (a.dens) {
-    const double w = trapw(col, a.nv);
     double* out = a.dens + (size_t)c.bid * a.nx;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      double s[2] = {w * (double)v[i].re, w * (double)v[i].im};
+      double s2[2] = {acc[2 * i], acc[2 * i + 1]};
 #pragma unroll
       for (int m = 1; m < C; m <<= 1) {
-        s[0] += c.shfl_xor(s[0], m, C);
-        s[1] += c.shfl_xor(s[1], m, C);
+        s2[0] += c.shfl_xor(s2[0], m, C);
+        s2[1] += c.shfl_xor(s2[1], m, C);
       }
       if (cc == 0) {
         const int j = t + i * TPT;
-        out[2 * j] = s[0];
-        out[2 * j + 1] = s[1];
+        out[2 * j] = s2[0];
+        out[2 * j + 1] = s2[1];
       }
     }
   }
@@ -302,32 +325,69 @@ struct FieldArgs {
   int ntiles;
   int nx;
   double scale;
+  double* rho_g;          // [nx] global scratch when the grid has several workgroups
+  unsigned int* counter;  // arrival counter (zero between launches)
   const double* okx;      // [nx]
-  const cx<double>* tw;   // exp(-2 pi i n / nx), n < nx
+  const cx<double>* tw;   // pow2 kernel: twiddle tables [lo | hi]; DFT kernel: exp(-2 pi i n / nx), n < nx
   const double* driver;   // [nx] or null
   double* e_out;          // [nx]
   double* e_out2;         // [nx] or null
   double* series;         // [2] or null: mean(E^2), mean(driver^2)
 };
 
-// reduce the partial sums into rho[x] = 1 - scale * sum (all FT threads)
+// rho[x] = 1 - scale * sum_tiles part[tile][x] for x in [x0, x0 + XS)
+// (all FT threads; XS divides FT or FT divides XS)
 template <int FT, class Ctx>
-VP_HD void field_density(Ctx& c, const FieldArgs& a, double* rho, double* tmp) {
+VP_HD void field_density(Ctx& c, const FieldArgs& a, int x0, int XS, double* rho, double* tmp) {
   const int nx = a.nx;
-  const int G = nx >= FT ? 1 : FT / nx;
-  for (int x0 = c.tid; x0 < nx * G; x0 += FT) {
-    const int x = x0 % nx, g = x0 / nx;
-    double s = 0.0;
-    for (int tile = g; tile < a.ntiles; tile += G) s += a.part[(size_t)tile * nx + x];
-    tmp[x0] = s;
+  const int G = XS >= FT ? 1 : FT / XS;
+  for (int l0 = c.tid; l0 < XS * G; l0 += FT) {
+    const int xl = l0 % XS, g = l0 / XS;
+    const double* p = a.part + x0 + xl;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int tile = g;
+    for (; tile + 3 * G < a.ntiles; tile += 4 * G) {
+      s0 += p[(size_t)tile * nx];
+      s1 += p[(size_t)(tile + G) * nx];
+      s2 += p[(size_t)(tile + 2 * G) * nx];
+      s3 += p[(size_t)(tile + 3 * G) * nx];
+    }
+    for (; tile < a.ntiles; tile += G) s0 += p[(size_t)tile * nx];
+    tmp[l0] = (s0 + s1) + (s2 + s3);
   }
   c.sync();
-  for (int x = c.tid; x < nx; x += FT) {
+  for (int xl = c.tid; xl < XS; xl += FT) {
     double s = 0.0;
-    for (int g = 0; g < G; ++g) s += tmp[g * nx + x];
-    rho[x] = 1.0 - a.scale * s;
+    for (int g = 0; g < G; ++g) s += tmp[g * XS + xl];
+    rho[xl] = 1.0 - a.scale * s;
   }
   c.sync();
+}
+
+// Several workgroups: each reduces an x slice into rho_g; the last one to
+// arrive (device-scope counter) continues with the whole rho.  Returns false
+// in the workgroups that are done.
+template <int FT, class Ctx>
+VP_HD bool field_gather(Ctx& c, const FieldArgs& a, double* rho, double* tmp, int* flag) {
+  const int nx = a.nx;
+  if (c.nblk == 1) {
+    field_density<FT>(c, a, 0, nx, rho, tmp);
+    return true;
+  }
+  const int XS = nx / c.nblk;
+  const int x0 = c.bid * XS;
+  field_density<FT>(c, a, x0, XS, rho, tmp);
+  for (int xl = c.tid; xl < XS; xl += FT) a.rho_g[x0 + xl] = rho[xl];
+  c.fence();
+  c.sync();
+  if (c.tid == 0) *flag = (c.arrive(a.counter) == (unsigned)(c.nblk - 1)) ? 1 : 0;
+  c.sync();
+  if (!*flag) return false;
+  c.fence();
+  for (int x = c.tid; x < nx; x += FT) rho[x] = c.load_coherent(a.rho_g + x);
+  if (c.tid == 0) c.reset(a.counter);
+  c.sync();
+  return true;
 }
 
 template <int FT, class Ctx>
@@ -345,7 +405,10 @@ struct FieldCfg {
   static constexpr int TPT = M / Q;
   static constexpr int nx = 2 * M;
   static constexpr int tmp_n = (nx > FT ? nx : FT);
-  static constexpr int lds = nx * 8 + ((Padded<M>::size * 16 > tmp_n * 8) ? Padded<M>::size * 16 : tmp_n * 8) + 64;
+  static constexpr int tw_bytes = TwSize<nx>::total * 16;
+  static constexpr int work = (Padded<M>::size * 16 > tmp_n * 8) ? Padded<M>::size * 16 : tmp_n * 8;
+  // [twiddles | okx (nx) | rho (nx) | fft buffer / reduction scratch | row_sum scratch | flag]
+  static constexpr int lds = tw_bytes + 2 * nx * 8 + work + 64 + 16;
 };
 
 template <int M, int Q, int FT, class Ctx>
@@ -353,11 +416,18 @@ VP_HD void field_body(Ctx& c, unsigned char* lds, const FieldArgs& a) {
   using Cfg = FieldCfg<M, Q, FT>;
   constexpr int TPT = Cfg::TPT;
   static_assert(TPT <= FT, "field transform needs at most FT threads");
-  double* rho = reinterpret_cast<double*>(lds);
-  cx<double>* buf = reinterpret_cast<cx<double>*>(lds + Cfg::nx * 8);
+  cx<double>* twl = reinterpret_cast<cx<double>*>(lds);
+  double* okx = reinterpret_cast<double*>(lds + Cfg::tw_bytes);
+  double* rho = okx + Cfg::nx;
+  cx<double>* buf = reinterpret_cast<cx<double>*>(rho + Cfg::nx);
   double* tmp = reinterpret_cast<double*>(buf);
-  double* rs = reinterpret_cast<double*>(lds + Cfg::lds - 64);
-  field_density<FT>(c, a, rho, tmp);
+  double* rs = reinterpret_cast<double*>(lds + Cfg::lds - 80);
+  int* flag = reinterpret_cast<int*>(lds + Cfg::lds - 16);
+  if (!field_gather<FT>(c, a, rho, tmp, flag)) return;
+  tw_stage<double, Cfg::nx>(c, twl, a.tw);
+  for (int x = c.tid; x < Cfg::nx; x += FT) okx[x] = a.okx[x];
+  c.sync();
+  const Tw<double> tw{twl};
   const bool act = c.tid < TPT;
   const int t = act ? c.tid : 0;
   cx<double> v[Q];
@@ -367,9 +437,9 @@ VP_HD void field_body(Ctx& c, unsigned char* lds, const FieldArgs& a) {
     v[i] = mk<double>(rho[2 * j], rho[2 * j + 1]);
   }
   PoissonMul<double> mul;
-  mul.okx = a.okx;
+  mul.okx = okx;
   mul.M = M;
-  spectral_apply<double, M, TPT>(c, v, buf, a.tw, t, mul, act);
+  spectral_apply<double, M, TPT>(c, v, buf, tw, t, mul, act);
   double acc[2] = {0.0, 0.0};
   if (act) {
 #pragma unroll
@@ -396,7 +466,7 @@ VP_HD void field_dft_body(Ctx& c, unsigned char* lds, const FieldArgs& a) {
   cx<double>* F = reinterpret_cast<cx<double>*>(lds + ((nx * 8 + 15) & ~15));
   double* tmp = reinterpret_cast<double*>(F);
   double* rs = reinterpret_cast<double*>(reinterpret_cast<unsigned char*>(F) + (size_t)(nx > FT ? nx : FT) * 16);
-  field_density<FT>(c, a, rho, tmp);
+  field_density<FT>(c, a, 0, nx, rho, tmp);
   for (int m = c.tid; m < nx; m += FT) {
     cx<double> s = mk<double>(0.0, 0.0);
     for (int j = 0; j < nx; ++j) {
