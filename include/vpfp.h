@@ -113,6 +113,17 @@ int vpfp_collide(vpfp_plan_t plan, const void* f_in, void* f_out, void* stream);
 int vpfp_step_n(vpfp_plan_t plan, void* f, double* e, int n_steps, const double* driver, double* e_hist,
                 double* moments, double* series, void* stream);
 
+/* Trapz moments of f per x after the step: moments [VPFP_NMOMENTS][nx] =
+ * n, j, T, q, fv4, vN (integrals of f v^p, p = 0..5) then f^2 and f log f.
+ * Replaces update_fields / the f-integrals of update_series, step.py:153-224. */
+int vpfp_moments(vpfp_plan_t plan, const void* f, double* moments, void* stream);
+
+/* First n_modes Fourier modes along x of every v column, complex128
+ * out [n_modes][nv] (interleaved re, im): sum_x f(x, v) exp(-2 pi i m x / nx).
+ * Replaces the "k0..." f-storage rule, step.py:130-135 (the reference keeps
+ * them as complex64). */
+int vpfp_x_modes(vpfp_plan_t plan, const void* f, int n_modes, double* out, void* stream);
+
 /* Instrumentation (no counterpart in the reference, which only logs wall
  * time per step to mlflow, manager.py:105-172).  With profiling on,
  * vpfp_step_n brackets each kernel launch with HIP events on its stream and

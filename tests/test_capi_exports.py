@@ -1,0 +1,42 @@
+"""The C-ABI library loads without a GPU and exports every entry point that
+include/vpfp.h declares (no compute call is made)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "vpfp.h")
+LIB = os.path.join(ROOT, "vlapy_amd", "libvpfp.so")
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(vpfp_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_boundary():
+    names = declared()
+    for must in ("vpfp_plan_create", "vpfp_plan_destroy", "vpfp_vdfdx", "vpfp_edfdv", "vpfp_field",
+                 "vpfp_solve_for_field", "vpfp_compute_charges", "vpfp_collide", "vpfp_collision_diagonals",
+                 "vpfp_tridiag_solve", "vpfp_step_n", "vpfp_moments", "vpfp_x_modes", "vpfp_last_error"):
+        assert must in names
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libvpfp.so not built (run __graft_entry__.build())")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared() if not hasattr(lib, n)]
+    assert not missing, missing
+    lib.vpfp_version.restype = ctypes.c_int
+    assert lib.vpfp_version() >= 1
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libvpfp.so not built")
+def test_python_binding_covers_the_header():
+    from vlapy_amd import _lib
+
+    assert set(declared()) <= set(_lib.SIGNATURES)

@@ -1,0 +1,102 @@
+"""Whole-run checks on the GPU at the benchmark configurations' sizes.
+
+* Collisionless Landau damping, nx=64 on [0, 2 pi/0.3), nv=512 on
+  [-6.4, 6.4], f0 = (1 + 1e-4 cos kx) exp(-v^2/2)/sqrt(2 pi), e0 = 0,
+  dt=0.1, 1000 leapfrog steps: the E(x, t) history against the oracle's, and
+  the damping rate and frequency of <E^2> against the oracle's (same
+  estimator, 0.5 % bar) and the Landau root (Canosa: 1.1598 - 0.012623i).
+* The collisional EPW (nx=64, nv=1024, Dougherty nu=1e-3, dt=0.05): the
+  first 200 steps against the oracle in FP64, the FP32 path's error stated.
+* 1024 x 8192 grid: size-independent properties (density conservation by
+  the advection and collision steps, kick/un-kick round trip).
+"""
+
+import numpy as np
+import pytest
+
+from oracle import vpfp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
+
+
+def run_gpu(g, f0, n, dt, nu=0.0, op="lb", precision="fp64"):
+    import torch
+
+    from vlapy_amd import ops
+    from vlapy_amd._lib import Plan
+
+    nx, nv = f0.shape
+    p = Plan(nx, nv, precision, g.kx, g.one_over_kx, g.v, g.kv, g.dv, nu, dt, op)
+    f = torch.as_tensor(f0, device="cuda").to(ops.f_dtype(p)).contiguous()
+    e = torch.zeros(nx, dtype=torch.float64, device="cuda")
+    eh = torch.zeros(n, nx, dtype=torch.float64, device="cuda")
+    mo = torch.zeros(n, 8, nx, dtype=torch.float64, device="cuda")
+    ops.step_n(p, f, e, n, None, eh, mo)
+    return f.double().cpu().numpy(), eh.cpu().numpy(), mo.cpu().numpy()
+
+
+def test_landau_damping_64x512_1000_steps():
+    k, nx, nv, dt, n = 0.3, 64, 512, 0.1, 1000
+    g = O.Grid.make(nx, nv, 2 * np.pi / k, 6.4)
+    f0 = O.perturbed_maxwellian(g.x, g.v, k, 1e-4)
+    f_g, eh_g, _ = run_gpu(g, f0, n, dt)
+    _, f_o, eh_o = O.run_steps(g, np.zeros(nx), f0, n, dt)
+    err_e = rel(eh_g, eh_o)
+    print("E history rel err over 1000 steps: %.3e, f: %.3e" % (err_e, rel(f_g, f_o)))
+    assert err_e < 1e-10
+    assert rel(f_g, f_o) < 1e-12
+    t = dt * (np.arange(n) + 1)
+    gg, wg = O.damping_rate_and_frequency(np.mean(eh_g ** 2, axis=1), t, t_min=8.0)
+    go, wo = O.damping_rate_and_frequency(np.mean(eh_o ** 2, axis=1), t, t_min=8.0)
+    assert abs(gg / go - 1) < 5e-3 and abs(wg / wo - 1) < 5e-3
+    assert gg == pytest.approx(-0.012623, rel=0.02) and wg == pytest.approx(1.1598, rel=0.005)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp32", 5e-3)])
+def test_collisional_epw_first_200_steps(precision, tol):
+    k, nx, nv, dt, n = 0.3, 64, 1024, 0.05, 200
+    g = O.Grid.make(nx, nv, 2 * np.pi / k, 6.4)
+    f0 = O.perturbed_maxwellian(g.x, g.v, k, 0.05)
+    f_g, eh_g, mo_g = run_gpu(g, f0, n, dt, 1e-3, "dg", precision)
+    _, f_o, eh_o = O.run_steps(g, np.zeros(nx), f0, n, dt, nu=1e-3, fp_type="dg")
+    print("%s: E rel err %.3e, f rel err %.3e" % (precision, rel(eh_g, eh_o), rel(f_g, f_o)))
+    assert rel(eh_g, eh_o) < tol
+    assert rel(f_g, f_o) < tol
+    n_drift = np.abs(mo_g[-1, 0].mean() / mo_g[0, 0].mean() - 1)
+    print("%s: relative drift of the mean density over %d steps %.3e" % (precision, n, n_drift))
+    assert n_drift < (1e-12 if precision == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_large_grid_invariants(precision):
+    import torch
+
+    from vlapy_amd import ops
+    from vlapy_amd._lib import Plan
+
+    nx, nv, k = 1024, 8192, 0.35
+    g = O.Grid.make(nx, nv, 2 * np.pi / k, 8.0)
+    rng = np.random.default_rng(4)
+    ph = rng.uniform(0, 2 * np.pi, 3)
+    f0 = O.perturbed_maxwellian(g.x, g.v, k, 0.1, harmonics=[(1, ph[0]), (2, ph[1]), (3, ph[2])])
+    p = Plan(nx, nv, precision, g.kx, g.one_over_kx, g.v, g.kv, g.dv, 1e-4, 0.02, "dg")
+    dtp = ops.f_dtype(p)
+    f = torch.as_tensor(f0, device="cuda").to(dtp).contiguous()
+    tol = 1e-12 if precision == "fp64" else 3e-6
+    # x-advection conserves every column's x-sum
+    out = ops.vdfdx(p, f, torch.empty_like(f), 0.02)
+    np.testing.assert_allclose(out.double().sum(0).cpu().numpy(), f.double().sum(0).cpu().numpy(),
+                               rtol=tol, atol=tol * 1e-3)
+    # kick then un-kick returns f
+    e = torch.as_tensor(0.3 * np.sin(k * g.x), device="cuda")
+    back = ops.edfdv(p, ops.edfdv(p, f, torch.empty_like(f), e, 0.02), torch.empty_like(f), e, -0.02)
+    assert rel(back.double().cpu().numpy(), f0) < (1e-12 if precision == "fp64" else 2e-6)
+    # the collision step conserves density per x (discrete identity, tails ~1e-14)
+    n0 = ops.compute_charges(p, f, torch.empty(nx, dtype=torch.float64, device="cuda"))
+    fc = ops.collide(p, f, torch.empty_like(f))
+    n1 = ops.compute_charges(p, fc, torch.empty(nx, dtype=torch.float64, device="cuda"))
+    assert rel(n1.cpu().numpy(), n0.cpu().numpy()) < (1e-12 if precision == "fp64" else 2e-6)

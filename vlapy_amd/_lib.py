@@ -53,6 +53,8 @@ SIGNATURES = {
     "vpfp_tridiag_solve": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
     "vpfp_collide": [_c_p, _c_p, _c_p, _c_p],
     "vpfp_step_n": [_c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p],
+    "vpfp_moments": [_c_p, _c_p, _c_p, _c_p],
+    "vpfp_x_modes": [_c_p, _c_p, _c_i, _c_p, _c_p],
     "vpfp_plan_set_profiling": [_c_p, _c_i],
     "vpfp_plan_kernel_times": [_c_p, ctypes.POINTER(_c_d), ctypes.POINTER(ctypes.c_longlong)],
     "vpfp_last_error": [],

@@ -92,6 +92,13 @@ __global__ __launch_bounds__(256) void k_diag(const T* f, int nx, int nv, const 
   vp::diag_body<T>(c, f, nx, nv, v, dv, nu_dt, op_dg, A, B, Cc);
 }
 
+template <class T>
+__global__ __launch_bounds__(256) void k_xmodes(const T* f, int nx, int nv, int nm, const cx<double>* tw,
+                                                cx<double>* out) {
+  VPFP_CTX;
+  vp::xmodes_body<T>(c, f, nx, nv, nm, tw, out);
+}
+
 template <class T, int M, int Q>
 __global__ __launch_bounds__((vp::RowCfg<T, M, Q>::wg)) void k_tri(vp::TriArgs<T> a) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -708,6 +715,38 @@ int vpfp_collide(vpfp_plan_t p, const void* fin, void* fout, void* stream) {
   }
   if (int rc = need_spectral_v(p)) return rc;
   return p->prec == VPFP_F64 ? do_collide<double>(p, fin, fout, S(stream)) : do_collide<float>(p, fin, fout, S(stream));
+}
+
+int vpfp_moments(vpfp_plan_t p, const void* f, double* moments, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!f || !moments) return fail(VPFP_EBADARG, "null array");
+  if (int rc = need_spectral_v(p)) return rc;
+  if (p->prec == VPFP_F64) {
+    auto a = row_args<double>(p, f, nullptr);
+    a.moments = moments;
+    a.flags = vp::RK_MOMENTS;
+    return row_dispatch<double>(a, S(stream));
+  }
+  auto a = row_args<float>(p, f, nullptr);
+  a.moments = moments;
+  a.flags = vp::RK_MOMENTS;
+  return row_dispatch<float>(a, S(stream));
+}
+
+int vpfp_x_modes(vpfp_plan_t p, const void* f, int n_modes, double* out, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!f || !out) return fail(VPFP_EBADARG, "null array");
+  if (n_modes < 1 || n_modes > p->nx) return fail(VPFP_EBADARG, "n_modes must be in [1, nx]");
+  const int total = n_modes * p->nv;
+  const dim3 grid((total + 255) / 256);
+  auto* o = reinterpret_cast<cx<double>*>(out);
+  if (p->prec == VPFP_F64)
+    hipLaunchKernelGGL(k_xmodes<double>, grid, dim3(256), 0, S(stream), reinterpret_cast<const double*>(f), p->nx,
+                       p->nv, n_modes, p->tw_xd, o);
+  else
+    hipLaunchKernelGGL(k_xmodes<float>, grid, dim3(256), 0, S(stream), reinterpret_cast<const float*>(f), p->nx,
+                       p->nv, n_modes, p->tw_xd, o);
+  return launched();
 }
 
 int vpfp_plan_set_profiling(vpfp_plan_t p, int on) {

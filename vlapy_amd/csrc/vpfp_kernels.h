@@ -99,7 +99,7 @@ VP_HD void coll_moments(Ctx& c, const double* d, double v0, double dvv, int N, d
 template <class T>
 struct RowArgs {
   const T* f_in;
-  T* f_out;
+  T* f_out;          // null: f is not written back (moments only)
   int nx, nv;
   const double* e;  // [nx] field for the kicks
   double dt_a, dt_b;
@@ -227,7 +227,7 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
   if (a.flags & RK_KICK_B)
     spectral_apply<T, M, TPT>(c, v, buf, tw, t, PhaseMul<T>((a.dk * a.dt_b) * e, M, t, TPT));
 
-  if (valid) {
+  if (valid && a.f_out) {
     cx<T>* dst = reinterpret_cast<cx<T>*>(a.f_out + (size_t)row * a.nv);
 #pragma unroll
     for (int i = 0; i < Q; ++i) dst[t + i * TPT] = v[i];
@@ -512,6 +512,26 @@ VP_HD void density_body(Ctx& c, const T* f, int nx, int nv, double scale, double
 #pragma unroll
   for (int k = 1; k < 64; k <<= 1) s[0] += c.shfl_xor(s[0], k, 64);
   if (row < nx && lane == 0) part[row] = scale * s[0];
+}
+
+// ---------------------------------------------------------------------------
+// first nm Fourier modes along x of every v column:
+//   out[m][v] = sum_x f(x, v) exp(-2 pi i m x / nx)     (step.py:130-135)
+// one thread per (m, v); the x sum is sequential in x order
+// ---------------------------------------------------------------------------
+template <class T, class Ctx>
+VP_HD void xmodes_body(Ctx& c, const T* f, int nx, int nv, int nm, const cx<double>* tw, cx<double>* out) {
+  const int idx = c.bid * c.nthr + c.tid;
+  if (idx >= nm * nv) return;
+  const int m = idx / nv, col = idx % nv;
+  double re = 0.0, im = 0.0;
+  for (int x = 0; x < nx; ++x) {
+    const cx<double> w = tw[(int)(((long long)m * x) % nx)];
+    const double fv = (double)f[(size_t)x * nv + col];
+    re += fv * w.re;
+    im += fv * w.im;
+  }
+  out[idx] = mk<double>(re, im);
 }
 
 // ---------------------------------------------------------------------------

@@ -101,3 +101,19 @@ def step_n(plan, f, e, n_steps, driver=None, e_hist=None, moments=None, series=N
     _lib.check(_lib.library().vpfp_step_n(plan.handle, _f(plan, f), _x(plan, e, "e"), n, drv, eh, mo, se,
                                           _stream()))
     return f, e
+
+
+def moments(plan, f, out):
+    """Trapz moments [8][nx]: n, j, T, q, fv4, vN, f^2, f log f."""
+    _lib.check(_lib.library().vpfp_moments(plan.handle, _f(plan, f),
+                                           _check(out, (_lib.NMOMENTS, plan.nx), torch.float64, "moments"),
+                                           _stream()))
+    return out
+
+
+def x_modes(plan, f, n_modes, out):
+    """First n_modes Fourier modes along x, complex128 [n_modes][nv]."""
+    _lib.check(_lib.library().vpfp_x_modes(plan.handle, _f(plan, f), int(n_modes),
+                                           _check(out, (int(n_modes), plan.nv), torch.complex128, "modes"),
+                                           _stream()))
+    return out
