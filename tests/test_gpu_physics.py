@@ -66,9 +66,14 @@ def test_collisional_epw_first_200_steps(precision, tol):
     print("%s: E rel err %.3e, f rel err %.3e" % (precision, rel(eh_g, eh_o), rel(f_g, f_o)))
     assert rel(eh_g, eh_o) < tol
     assert rel(f_g, f_o) < tol
-    n_drift = np.abs(mo_g[-1, 0].mean() / mo_g[0, 0].mean() - 1)
-    print("%s: relative drift of the mean density over %d steps %.3e" % (precision, n, n_drift))
-    assert n_drift < (1e-12 if precision == "fp64" else 1e-6)
+    # the discrete Dougherty step itself changes the density at the velocity
+    # boundary (f ~ 1e-9 there); the GPU must reproduce the oracle's drift
+    n_o = O.compute_charges(f_o, g.dv).mean()
+    n_0 = O.compute_charges(f0, g.dv).mean()
+    d_gpu = mo_g[-1, 0].mean() / n_0 - 1
+    d_ora = n_o / n_0 - 1
+    print("%s: relative drift of the mean density over %d steps: gpu %.3e oracle %.3e" % (precision, n, d_gpu, d_ora))
+    assert abs(d_gpu - d_ora) < (1e-13 if precision == "fp64" else 1e-6)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])

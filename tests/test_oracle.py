@@ -351,3 +351,15 @@ def test_golden_timestep_and_storage(gold):
         np.testing.assert_allclose(temp["fields"][k], z["ts_fields_" + k], rtol=1e-12, atol=atol, err_msg=k)
     for k in O.SERIES_KEYS:
         np.testing.assert_allclose(temp["series"][k], z["ts_series_" + k], rtol=1e-12, atol=1e-13, err_msg=k)
+
+
+def test_reference_damping_rate_estimator_closed_form():
+    """low_level_helpers.py:69-87 on a travelling wave A e^{g t} sin(kx - w t):
+    |E_k1| = A e^{g t} nx/2, so the mean slope of log|E_k1| is g exactly."""
+    nx, k, w, g = 32, 0.3, 1.16, -0.0126
+    x = np.linspace(0, 2 * np.pi / k, nx, endpoint=False)
+    t = np.arange(400) * 0.1
+    e = 1e-3 * np.exp(g * t)[:, None] * np.sin(k * x[None, :] - w * t[:, None])
+    assert O.damping_rate_reference(e, t) == pytest.approx(g, rel=1e-10)
+    wrong = 1e-3 * np.exp(g * t)[:, None] * np.sin(2 * k * x[None, :] - w * t[:, None])
+    assert not np.isfinite(O.damping_rate_reference(wrong, t)) or abs(O.damping_rate_reference(wrong, t) - g) > 1e-3
