@@ -124,6 +124,32 @@ int vpfp_moments(vpfp_plan_t plan, const void* f, double* moments, void* stream)
  * them as complex64). */
 int vpfp_x_modes(vpfp_plan_t plan, const void* f, int n_modes, double* out, void* stream);
 
+/* ---- Batched parameter scan (no counterpart in the reference, which runs
+ * one instance per process).  A batch holds n_inst instances of one size
+ * class (nx, nv) with their own box length (kx, one_over_kx per instance)
+ * and collision frequency, sharing v, dt and the operator.  f is
+ * [n_inst][nx][nv], e [n_inst][nx]; each step is the leapfrog step of
+ * vpfp_step_n followed by the collision step, with no storage moments.
+ * series [n_steps][n_inst][2] (optional) receives mean E^2 and 0. */
+typedef struct vpfp_batch_s* vpfp_batch_t;
+typedef struct {
+  int n_inst;
+  int nx;                     /* power of two in [4, 256] */
+  int nv;                     /* power of two in [16, 8192] */
+  int precision;              /* VPFP_F64 or VPFP_F32 */
+  const double* kx;           /* [n_inst][nx] */
+  const double* one_over_kx;  /* [n_inst][nx] */
+  const double* v;            /* [nv] */
+  const double* kv;           /* [nv] */
+  double dv;
+  const double* nu;           /* [n_inst] */
+  double dt;
+  int op;
+} vpfp_batch_desc;
+int vpfp_batch_create(const vpfp_batch_desc* desc, vpfp_batch_t* out);
+int vpfp_batch_step_n(vpfp_batch_t batch, void* f, double* e, int n_steps, double* series, void* stream);
+int vpfp_batch_destroy(vpfp_batch_t batch);
+
 /* Instrumentation (no counterpart in the reference, which only logs wall
  * time per step to mlflow, manager.py:105-172).  With profiling on,
  * vpfp_step_n brackets each kernel launch with HIP events on its stream and

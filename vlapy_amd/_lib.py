@@ -40,6 +40,23 @@ class GridDesc(ctypes.Structure):
     ]
 
 
+class BatchDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_inst", _c_i),
+        ("nx", _c_i),
+        ("nv", _c_i),
+        ("precision", _c_i),
+        ("kx", ctypes.POINTER(_c_d)),
+        ("one_over_kx", ctypes.POINTER(_c_d)),
+        ("v", ctypes.POINTER(_c_d)),
+        ("kv", ctypes.POINTER(_c_d)),
+        ("dv", _c_d),
+        ("nu", ctypes.POINTER(_c_d)),
+        ("dt", _c_d),
+        ("op", _c_i),
+    ]
+
+
 # name -> (argtypes); every function returns int status unless listed in _OTHER
 SIGNATURES = {
     "vpfp_plan_create": [ctypes.POINTER(GridDesc), ctypes.POINTER(_c_p)],
@@ -55,6 +72,9 @@ SIGNATURES = {
     "vpfp_step_n": [_c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p],
     "vpfp_moments": [_c_p, _c_p, _c_p, _c_p],
     "vpfp_x_modes": [_c_p, _c_p, _c_i, _c_p, _c_p],
+    "vpfp_batch_create": [ctypes.POINTER(BatchDesc), ctypes.POINTER(_c_p)],
+    "vpfp_batch_step_n": [_c_p, _c_p, _c_p, _c_i, _c_p, _c_p],
+    "vpfp_batch_destroy": [_c_p],
     "vpfp_plan_set_profiling": [_c_p, _c_i],
     "vpfp_plan_kernel_times": [_c_p, ctypes.POINTER(_c_d), ctypes.POINTER(ctypes.c_longlong)],
     "vpfp_last_error": [],

@@ -35,7 +35,7 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 struct GpuCtx {
-  int tid, nthr, bid, nblk;
+  int tid, nthr, bid, nblk, bidy;
   __device__ __forceinline__ void sync() { __syncthreads(); }
   __device__ __forceinline__ void fence() { __threadfence(); }
   __device__ __forceinline__ unsigned arrive(unsigned* c) { return atomicAdd(c, 1u); }
@@ -47,7 +47,7 @@ struct GpuCtx {
   __device__ __forceinline__ float shfl_xor(float v, int m, int w) { return __shfl_xor(v, m, w); }
 };
 
-#define VPFP_CTX GpuCtx c{(int)threadIdx.x, (int)blockDim.x, (int)blockIdx.x, (int)gridDim.x}
+#define VPFP_CTX GpuCtx c{(int)threadIdx.x, (int)blockDim.x, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y}
 
 // ------------------------------------------------------------------ kernels
 template <class T, int M, int Q>
@@ -131,15 +131,25 @@ int drift_grid(int ntiles) { return ntiles < kDriftMaxWG ? ntiles : kDriftMaxWG;
 // field workgroups: one per 64 x values (the last to finish solves Poisson)
 int field_grid(int nx) { return nx >= 128 ? nx / 64 : 1; }
 
-template <class T, int M>
-int launch_row(const vp::RowArgs<T>& a, hipStream_t s) {
-  constexpr int Q = qfor(M);
+template <class T, int M, int Q>
+int launch_row_q(const vp::RowArgs<T>& a, hipStream_t s) {
   using Cfg = vp::RowCfg<T, M, Q>;
   auto kern = k_row<T, M, Q>;
   if (int rc = set_lds(kern, Cfg::lds)) return rc;
   const int grid = (a.nx + Cfg::rows - 1) / Cfg::rows;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(Cfg::wg), Cfg::lds, s, a);
   return launched();
+}
+
+// Q = 8 values per thread fills the GPU when there are many rows; with few
+// rows (fewer than ~16k threads at Q = 8) Q = 4 spreads each row over twice
+// as many threads and shortens the per-row latency chain.
+template <class T, int M>
+int launch_row(const vp::RowArgs<T>& a, hipStream_t s) {
+  if constexpr (M >= 16 && M <= 2048) {
+    if ((long long)a.nx * (M / 8) < 16384) return launch_row_q<T, M, 4>(a, s);
+  }
+  return launch_row_q<T, M, qfor(M)>(a, s);
 }
 
 template <class T, int M>
@@ -154,21 +164,22 @@ int launch_tri(const vp::TriArgs<T>& a, hipStream_t s) {
 }
 
 template <class T, int M, int C>
-int launch_drift_c(const vp::DriftArgs<T>& a, hipStream_t s) {
-  constexpr int Q = qfor(M);
+int launch_drift_c(const vp::DriftArgs<T>& a, hipStream_t s, int gy) {
+  constexpr int Q = M <= 64 ? (M < 4 ? M : 4) : 8;
   using Cfg = vp::DriftCfg<T, M, Q, C>;
   auto kern = k_drift<T, M, Q, C>;
+  if (a.fuse_field && !Cfg::tail) return fail(VPFP_EUNSUPPORTED, "field tail not available at this size");
   if (int rc = set_lds(kern, Cfg::lds)) return rc;
-  hipLaunchKernelGGL(kern, dim3(drift_grid(a.nv / C)), dim3(Cfg::wg), Cfg::lds, s, a);
+  hipLaunchKernelGGL(kern, dim3(gy > 1 ? a.nv / C : drift_grid(a.nv / C), gy), dim3(Cfg::wg), Cfg::lds, s, a);
   return launched();
 }
 
 template <class T, int M>
-int launch_drift(const vp::DriftArgs<T>& a, int C, hipStream_t s) {
+int launch_drift(const vp::DriftArgs<T>& a, int C, hipStream_t s, int gy) {
   switch (C) {
-    case 4: return launch_drift_c<T, M, 4>(a, s);
-    case 8: return launch_drift_c<T, M, 8>(a, s);
-    case 16: return launch_drift_c<T, M, 16>(a, s);
+    case 4: return launch_drift_c<T, M, 4>(a, s, gy);
+    case 8: return launch_drift_c<T, M, 8>(a, s, gy);
+    case 16: return launch_drift_c<T, M, 16>(a, s, gy);
   }
   return fail(VPFP_EUNSUPPORTED, "drift tile width");
 }
@@ -221,9 +232,9 @@ int tri_dispatch(const vp::TriArgs<T>& a, hipStream_t s) {
 }
 
 template <class T>
-int drift_dispatch(const vp::DriftArgs<T>& a, int C, hipStream_t s) {
+int drift_dispatch(const vp::DriftArgs<T>& a, int C, hipStream_t s, int gy = 1) {
   const int M = a.nx / 2;
-#define CALL(m) launch_drift<T, ((m) <= 1024 ? (m) : 2)>(a, C, s)
+#define CALL(m) launch_drift<T, ((m) <= 1024 ? (m) : 2)>(a, C, s, gy)
   VPFP_SWITCH_M(M, CALL, 1024)
 #undef CALL
   return fail(VPFP_EUNSUPPORTED, "nx must be a power of two in [4, 2048] for the x-advection");
@@ -368,7 +379,7 @@ int make_tables(vpfp_plan_s* p) {
 
 int drift_tile(int nx, int nv) {
   const int M = nx / 2;
-  int C = M <= 128 ? 16 : (M == 256 ? 8 : 4);
+  int C = M <= 256 ? 16 : (M == 512 ? 8 : 4);
   while (C > nv) C /= 2;
   return C >= 4 ? C : 0;
 }
@@ -447,6 +458,10 @@ vp::DriftArgs<T> drift_args(vpfp_plan_t p, const void* fin, void* fout, double d
   a.v = p->v;
   a.tw = reinterpret_cast<const cx<T>*>(p->tw_x);
   a.dens = dens;
+  a.fuse_field = 0;
+  std::memset(&a.fld, 0, sizeof(a.fld));
+  a.dk_inst = nullptr;
+  a.series_stride = 0;
   return a;
 }
 
@@ -550,12 +565,21 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
   ra.flags = vp::RK_KICK_A;
   if (n > 0) PROF(KK_ROW, row_dispatch<T>(ra, s));
   const double* e_cur = e;
+  // small grids: the last drift workgroup solves the field (one launch less)
+  const bool fuse = p->nx <= 256;
   for (int i = 0; i < n; ++i) {
-    PROF(KK_DRIFT, (do_vdfdx<T>(p, f, f, p->dt, p->dens, s)));
     double* ei = e_hist ? e_hist + (size_t)i * p->nx : p->ering + (size_t)(i & 1) * p->nx;
     auto fa = field_args(p, p->dens, p->ntiles, p->dv, driver ? driver + (size_t)i * p->nx : nullptr, ei, nullptr,
                          series ? series + 2 * (size_t)i : nullptr, true);
-    PROF(KK_FIELD, field_dispatch(fa, true, s));
+    if (fuse) {
+      auto da = drift_args<T>(p, f, f, p->dt, p->dens);
+      da.fuse_field = 1;
+      da.fld = fa;
+      PROF(KK_DRIFT, drift_dispatch<T>(da, p->C, s));
+    } else {
+      PROF(KK_DRIFT, (do_vdfdx<T>(p, f, f, p->dt, p->dens, s)));
+      PROF(KK_FIELD, field_dispatch(fa, true, s));
+    }
     e_cur = ei;
     auto rb = row_args<T>(p, f, f);
     rb.e = e_cur;
@@ -568,6 +592,110 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
   }
   if (n > 0) VPFP_HIP(hipMemcpyAsync(e, e_cur, sizeof(double) * p->nx, hipMemcpyDeviceToDevice, s));
   return prof_collect(p);
+}
+
+}  // namespace
+
+// ====================================================================== batched scan
+struct vpfp_batch_s {
+  int n = 0, nx = 0, nv = 0, prec = VPFP_F64, op = VPFP_OP_DG, C = 0, gx = 0;
+  double dv = 0, dt = 0, v0 = 0, dvv = 0, dk_v = 0;
+  double *dk_x = nullptr, *okx = nullptr, *nu_dt = nullptr, *v = nullptr;
+  void *tw_v = nullptr, *tw_x = nullptr;
+  cx<double>* tw_xs = nullptr;
+  double* dens = nullptr;
+  unsigned int* counters = nullptr;
+  double* ering = nullptr;
+  std::vector<void*> allocs;
+  ~vpfp_batch_s() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+template <class P>
+int balloc(vpfp_batch_s* b, P** out, size_t bytes) {
+  void* ptr = nullptr;
+  hipError_t e = hipMalloc(&ptr, bytes > 0 ? bytes : 16);
+  if (e != hipSuccess) return fail(VPFP_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  b->allocs.push_back(ptr);
+  *out = reinterpret_cast<P*>(ptr);
+  return VPFP_OK;
+}
+
+template <class T, class P>
+int bput(vpfp_batch_s* b, P** dst, const std::vector<T>& src) {
+  if (int rc = balloc(b, dst, sizeof(T) * src.size())) return rc;
+  return upload(*dst, src.data(), sizeof(T) * src.size());
+}
+
+template <class T>
+int batch_tables(vpfp_batch_s* b) {
+  cx<T>* tv = nullptr;
+  cx<T>* tx = nullptr;
+  if (int rc = bput(b, &tv, split_twiddles<T>(b->nv))) return rc;
+  if (int rc = bput(b, &tx, split_twiddles<T>(b->nx))) return rc;
+  if (int rc = bput(b, &b->tw_xs, split_twiddles<double>(b->nx))) return rc;
+  b->tw_v = tv;
+  b->tw_x = tx;
+  return VPFP_OK;
+}
+
+template <class T>
+int batch_step_n(vpfp_batch_s* b, void* f, double* e, int n, double* series, hipStream_t s) {
+  vp::RowArgs<T> ra;
+  std::memset(&ra, 0, sizeof(ra));
+  ra.f_in = reinterpret_cast<const T*>(f);
+  ra.f_out = reinterpret_cast<T*>(f);
+  ra.nx = b->n * b->nx;
+  ra.nv = b->nv;
+  ra.dk = b->dk_v;
+  ra.v0 = b->v0;
+  ra.dvv = b->dvv;
+  ra.dv = b->dv;
+  ra.tw = reinterpret_cast<const cx<T>*>(b->tw_v);
+  ra.op_dg = b->op == VPFP_OP_DG;
+  ra.nu_dt_inst = b->nu_dt;
+  ra.rows_per_inst = b->nx;
+  ra.dt_a = ra.dt_b = 0.5 * b->dt;
+  ra.e = e;
+  ra.flags = vp::RK_KICK_A;
+  if (n > 0)
+    if (int rc = row_dispatch<T>(ra, s)) return rc;
+  const double* e_cur = e;
+  for (int i = 0; i < n; ++i) {
+    double* ei = b->ering + (size_t)(i & 1) * b->n * b->nx;
+    vp::DriftArgs<T> da;
+    std::memset(&da, 0, sizeof(da));
+    da.f_in = reinterpret_cast<const T*>(f);
+    da.f_out = reinterpret_cast<T*>(f);
+    da.nx = b->nx;
+    da.nv = b->nv;
+    da.dt = b->dt;
+    da.v = b->v;
+    da.tw = reinterpret_cast<const cx<T>*>(b->tw_x);
+    da.dens = b->dens;
+    da.dk_inst = b->dk_x;
+    da.fuse_field = 1;
+    da.series_stride = 2;
+    da.fld.part = b->dens;
+    da.fld.ntiles = b->gx;
+    da.fld.nx = b->nx;
+    da.fld.scale = b->dv;
+    da.fld.counter = b->counters;
+    da.fld.okx = b->okx;
+    da.fld.tw = b->tw_xs;
+    da.fld.e_out = ei;
+    da.fld.series = series ? series + (size_t)i * b->n * 2 : nullptr;
+    if (int rc = drift_dispatch<T>(da, b->C, s, b->n)) return rc;
+    e_cur = ei;
+    ra.e = e_cur;
+    ra.flags = vp::RK_KICK_A | vp::RK_COLLIDE | (i + 1 < n ? vp::RK_KICK_B : 0);
+    if (int rc = row_dispatch<T>(ra, s)) return rc;
+  }
+  if (n > 0) VPFP_HIP(hipMemcpyAsync(e, e_cur, sizeof(double) * b->n * b->nx, hipMemcpyDeviceToDevice, s));
+  return VPFP_OK;
 }
 
 }  // namespace
@@ -747,6 +875,80 @@ int vpfp_x_modes(vpfp_plan_t p, const void* f, int n_modes, double* out, void* s
     hipLaunchKernelGGL(k_xmodes<float>, grid, dim3(256), 0, S(stream), reinterpret_cast<const float*>(f), p->nx,
                        p->nv, n_modes, p->tw_xd, o);
   return launched();
+}
+
+int vpfp_batch_create(const vpfp_batch_desc* d, vpfp_batch_t* out) {
+  if (!d || !out) return fail(VPFP_EBADARG, "null argument");
+  *out = nullptr;
+  if (d->n_inst < 1 || d->n_inst > 65535) return fail(VPFP_EBADARG, "n_inst must be in [1, 65535]");
+  if (!is_pow2(d->nx) || d->nx < 4 || d->nx > 256)
+    return fail(VPFP_EUNSUPPORTED, "batched scans need nx a power of two in [4, 256]");
+  if (!is_pow2(d->nv) || d->nv < 16 || d->nv > 8192)
+    return fail(VPFP_EUNSUPPORTED, "batched scans need nv a power of two in [16, 8192]");
+  if (d->precision != VPFP_F64 && d->precision != VPFP_F32) return fail(VPFP_EBADARG, "unknown precision");
+  if (d->op != VPFP_OP_LB && d->op != VPFP_OP_DG) return fail(VPFP_EUNSUPPORTED, "collision operator must be lb or dg");
+  if (!d->kx || !d->one_over_kx || !d->v || !d->kv || !d->nu) return fail(VPFP_EBADARG, "null array");
+  if (!(d->dv > 0.0)) return fail(VPFP_EBADARG, "dv must be positive");
+  if (!fftfreq_like(d->kv, d->nv)) return fail(VPFP_EUNSUPPORTED, "kv is not an fftfreq wavenumber table");
+  double v0 = 0.0, dvv = 0.0;
+  if (!uniform_grid(d->v, d->nv, &v0, &dvv)) return fail(VPFP_EUNSUPPORTED, "v is not a uniform grid");
+  std::vector<double> dk(d->n_inst), nudt(d->n_inst);
+  for (int i = 0; i < d->n_inst; ++i) {
+    if (!fftfreq_like(d->kx + (size_t)i * d->nx, d->nx))
+      return fail(VPFP_EUNSUPPORTED, "kx of instance " + std::to_string(i) + " is not an fftfreq table");
+    if (d->nu[i] < 0.0) return fail(VPFP_EUNSUPPORTED, "negative collision frequency");
+    dk[i] = d->kx[(size_t)i * d->nx + 1];
+    nudt[i] = d->nu[i] * d->dt;
+  }
+  auto* b = new vpfp_batch_s();
+  b->n = d->n_inst;
+  b->nx = d->nx;
+  b->nv = d->nv;
+  b->prec = d->precision;
+  b->op = d->op;
+  b->dv = d->dv;
+  b->dt = d->dt;
+  b->v0 = v0;
+  b->dvv = dvv;
+  b->dk_v = d->kv[1];
+  b->C = drift_tile(d->nx, d->nv);
+  b->gx = d->nv / b->C;
+  std::vector<double> vv(d->v, d->v + d->nv);
+  std::vector<double> okx(d->one_over_kx, d->one_over_kx + (size_t)d->n_inst * d->nx);
+  int rc = VPFP_OK;
+  if (!rc) rc = bput(b, &b->dk_x, dk);
+  if (!rc) rc = bput(b, &b->nu_dt, nudt);
+  if (!rc) rc = bput(b, &b->v, vv);
+  if (!rc) rc = bput(b, &b->okx, okx);
+  if (!rc) rc = b->prec == VPFP_F64 ? batch_tables<double>(b) : batch_tables<float>(b);
+  if (!rc) rc = balloc(b, &b->dens, sizeof(double) * (size_t)b->n * b->gx * b->nx);
+  if (!rc) rc = balloc(b, &b->counters, sizeof(unsigned) * 16 * (size_t)b->n);
+  if (!rc) rc = balloc(b, &b->ering, sizeof(double) * 2 * (size_t)b->n * b->nx);
+  if (!rc) {
+    hipError_t e = hipMemset(b->counters, 0, sizeof(unsigned) * 16 * (size_t)b->n);
+    if (e != hipSuccess) rc = fail(VPFP_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  }
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return VPFP_OK;
+}
+
+int vpfp_batch_destroy(vpfp_batch_t b) {
+  if (!b) return VPFP_OK;
+  (void)hipDeviceSynchronize();
+  delete b;
+  return VPFP_OK;
+}
+
+int vpfp_batch_step_n(vpfp_batch_t b, void* f, double* e, int n_steps, double* series, void* stream) {
+  if (!b) return fail(VPFP_EBADARG, "null batch");
+  if (!f || !e) return fail(VPFP_EBADARG, "null array");
+  if (n_steps < 0) return fail(VPFP_EBADARG, "negative step count");
+  return b->prec == VPFP_F64 ? batch_step_n<double>(b, f, e, n_steps, series, S(stream))
+                             : batch_step_n<float>(b, f, e, n_steps, series, S(stream));
 }
 
 int vpfp_plan_set_profiling(vpfp_plan_t p, int on) {

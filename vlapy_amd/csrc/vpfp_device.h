@@ -2,7 +2,8 @@
 //
 // Everything here is written against a small block context `Ctx` that
 // provides `tid`, `nthr` (thread index and count in the workgroup), `bid`,
-// `nblk` (workgroup index and count), `sync()` (workgroup barrier),
+// `nblk` (workgroup index and count along x), `bidy` (workgroup index along
+// y: the instance of a batched scan), `sync()` (workgroup barrier),
 // `shfl_xor(v, mask, width)` (lane exchange inside a wavefront) and, for the
 // cross-workgroup hand-off of the field kernel, `fence()` (device-scope
 // fence), `arrive(counter)` (device-scope atomic increment, returns the old
@@ -70,6 +71,23 @@ template <int M>
 struct Padded {
   static constexpr int size = M + ((M - 1) >> 4) + 1;
 };
+
+// ---------------------------------------------------------------------------
+// 1/x to double precision: hardware reciprocal estimate refined by two
+// Newton steps (within an ulp of the correctly rounded quotient; cheaper
+// than the IEEE division sequence in the tridiagonal sweeps)
+// ---------------------------------------------------------------------------
+VP_HD double rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+#else
+  return 1.0 / x;
+#endif
+}
 
 // ---------------------------------------------------------------------------
 // exp(-i theta) in the working precision
@@ -373,7 +391,7 @@ VP_HD void tridiag(Ctx& c, double* d, const Coef& cf, int t, double* red) {
     const int m = m0 + i;
     const double lm = cf.l(m), um = cf.u(m), bm = cf.b(m);
     const double den = (i == 1) ? bm : bm - lm * r;
-    const double inv = 1.0 / den;
+    const double inv = rcp(den);
     p = (i == 1) ? d[i] * inv : (d[i] - lm * p) * inv;
     q = (i == 1) ? -lm * inv : -lm * q * inv;
     r = um * inv;
@@ -385,7 +403,7 @@ VP_HD void tridiag(Ctx& c, double* d, const Coef& cf, int t, double* red) {
     const int m = m0 + i;
     const double lm = cf.l(m), um = cf.u(m), bm = cf.b(m);
     const double den = (i == L - 2) ? bm : bm - um * r2;
-    const double inv = 1.0 / den;
+    const double inv = rcp(den);
     p2 = (i == L - 2) ? d[i] * inv : (d[i] - um * p2) * inv;
     q2 = (i == L - 2) ? -um * inv : -um * q2 * inv;
     r2 = lm * inv;
@@ -415,13 +433,13 @@ VP_HD void tridiag(Ctx& c, double* d, const Coef& cf, int t, double* red) {
       const int im = i - s, ip = i + s;
       double na = 0.0, nb = rb[h], nc = 0.0, nd = rd[h];
       if (im >= 0) {
-        const double al = -ra[h] / B[im];
+        const double al = -ra[h] * rcp(B[im]);
         na = al * A[im];
         nb += al * C[im];
         nd += al * D[im];
       }
       if (ip < NR) {
-        const double ga = -rc[h] / B[ip];
+        const double ga = -rc[h] * rcp(B[ip]);
         nc = ga * C[ip];
         nb += ga * A[ip];
         nd += ga * D[ip];
@@ -433,8 +451,8 @@ VP_HD void tridiag(Ctx& c, double* d, const Coef& cf, int t, double* red) {
     }
     c.sync();
   }
-  const double xs = rd[0] / rb[0];
-  const double xe = rd[1] / rb[1];
+  const double xs = rd[0] * rcp(rb[0]);
+  const double xe = rd[1] * rcp(rb[1]);
   // interior rows m0+1 .. m0+L-2 with both chunk ends known (Thomas)
   double rr[L];
   d[0] = xs;
@@ -443,7 +461,7 @@ VP_HD void tridiag(Ctx& c, double* d, const Coef& cf, int t, double* red) {
     const int m = m0 + i;
     const double lm = cf.l(m), um = cf.u(m), bm = cf.b(m);
     const double den = (i == 1) ? bm : bm - lm * rr[i - 1];
-    const double inv = 1.0 / den;
+    const double inv = rcp(den);
     d[i] = (i == 1) ? (d[i] - lm * xs) * inv : (d[i] - lm * d[i - 1]) * inv;
     rr[i] = um * inv;
   }

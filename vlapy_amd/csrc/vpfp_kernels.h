@@ -111,6 +111,8 @@ struct RowArgs {
   int op_dg;
   double* moments;  // [kMoments][nx]
   int flags;
+  const double* nu_dt_inst;  // batched scan: nu*dt per instance (rows are instance-major), or null
+  int rows_per_inst;         // x rows per instance in a batch
 };
 
 template <class T, int M, int Q>
@@ -170,7 +172,8 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
     c.sync();
     const int N = a.nv;
     const int m0 = t * L;
-    if ((a.flags & RK_COLLIDE) && a.nu_dt > 0.0) {
+    const double nu_dt = a.nu_dt_inst ? a.nu_dt_inst[rowc / a.rows_per_inst] : a.nu_dt;
+    if ((a.flags & RK_COLLIDE) && nu_dt > 0.0) {
       double vbar, temp;
       coll_moments<TPT, L>(c, d, a.v0, a.dvv, N, a.dv, a.op_dg != 0, t, rs, vbar, temp);
       const double dv2 = a.dv * a.dv;
@@ -178,11 +181,11 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
       cf.v0 = a.v0;
       cf.dvv = a.dvv;
       cf.N = N;
-      cf.ndt = a.nu_dt;
+      cf.ndt = nu_dt;
       cf.A0 = -temp / dv2;
       cf.H = 0.5 / a.dv;
       cf.vbar = vbar;
-      cf.bd = 1.0 + a.nu_dt * (2.0 * temp / dv2);
+      cf.bd = 1.0 + nu_dt * (2.0 * temp / dv2);
       tridiag<TPT, L>(c, d, cf, t, red);
     }
     if (a.flags & RK_MOMENTS) {
@@ -207,7 +210,8 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
         p *= vm;
         s[5] += p;
         s[6] += wf * d[i];
-        s[7] += wf * log(d[i]);
+        // f log f: in the storage precision (FP32 log on the FP32 path)
+        s[7] += wf * (sizeof(T) == 8 ? log(d[i]) : (double)logf((float)d[i]));
       }
       row_sum<TPT, kMoments>(c, s, rs);
       if (valid && t == 0) {
@@ -235,89 +239,6 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
 }
 
 // ---------------------------------------------------------------------------
-// drift kernel (x-advection of C columns) + partial density sums
-// ---------------------------------------------------------------------------
-template <class T>
-struct DriftArgs {
-  const T* f_in;
-  T* f_out;
-  int nx, nv;
-  double dt;
-  double dk;         // kx[1] of the fftfreq table
-  const double* v;   // [nv]
-  const cx<T>* tw;   // twiddle tables of exp(-2 pi i n / nx), [lo | hi]
-  double* dens;      // [gridDim][nx] partial sums (unscaled), or null
-};
-
-template <class T, int M, int Q, int C>
-struct DriftCfg {
-  static constexpr int TPT = M / Q;
-  static constexpr int wg = TPT * C;
-  static constexpr int stride = Padded<M>::size;
-  static constexpr int tw_bytes = (TwSize<2 * M>::total * (int)sizeof(cx<T>) + 15) & ~15;
-  static constexpr int lds = tw_bytes + C * stride * (int)sizeof(cx<T>);
-};
-
-template <class T, int M, int Q, int C, class Ctx>
-VP_HD void drift_body(Ctx& c, unsigned char* lds, const DriftArgs<T>& a) {
-  using Cfg = DriftCfg<T, M, Q, C>;
-  constexpr int TPT = Cfg::TPT;
-  const int cc = c.tid % C;
-  const int t = c.tid / C;
-  cx<T>* twl = reinterpret_cast<cx<T>*>(lds);
-  cx<T>* buf = reinterpret_cast<cx<T>*>(lds + Cfg::tw_bytes) + cc * Cfg::stride;
-  const size_t nv = (size_t)a.nv;
-  tw_stage<T, 2 * M>(c, twl, a.tw);
-  const Tw<T> tw{twl};
-  double acc[2 * Q];
-#pragma unroll
-  for (int i = 0; i < 2 * Q; ++i) acc[i] = 0.0;
-  const int ntiles = a.nv / C;
-  // the grid strides over the column tiles; workgroup b accumulates the
-  // density of every tile it advects into one partial row dens[b][x]
-  for (int tile = c.bid; tile < ntiles; tile += c.nblk) {
-    const int col = tile * C + cc;
-    const T* fi = a.f_in + col;
-    cx<T> v[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const int j = t + i * TPT;
-      v[i] = mk<T>(fi[(size_t)(2 * j) * nv], fi[(size_t)(2 * j + 1) * nv]);
-    }
-    const double vc = a.v[col];
-    c.sync();  // twiddle tables staged (first tile) / buffer free (later tiles)
-    spectral_apply<T, M, TPT>(c, v, buf, tw, t, PhaseMul<T>((a.dk * a.dt) * vc, M, t, TPT));
-    T* fo = a.f_out + col;
-    const double w = trapw(col, a.nv);
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const int j = t + i * TPT;
-      fo[(size_t)(2 * j) * nv] = v[i].re;
-      fo[(size_t)(2 * j + 1) * nv] = v[i].im;
-      acc[2 * i] += w * (double)v[i].re;
-      acc[2 * i + 1] += w * (double)v[i].im;
-    }
-  }
-  if (a.dens) {
-    double* out = a.dens + (size_t)c.bid * a.nx;
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      double s2[2] = {acc[2 * i], acc[2 * i + 1]};
-#pragma unroll
-      for (int m = 1; m < C; m <<= 1) {
-        s2[0] += c.shfl_xor(s2[0], m, C);
-        s2[1] += c.shfl_xor(s2[1], m, C);
-      }
-      if (cc == 0) {
-        const int j = t + i * TPT;
-        out[2 * j] = s2[0];
-        out[2 * j + 1] = s2[1];
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
 // field kernel: n(x) = scale * sum_tiles part[tile][x];  E = Re ifft(i/kx fft(1 - n)) + driver
 // ---------------------------------------------------------------------------
 struct FieldArgs {
@@ -337,23 +258,23 @@ struct FieldArgs {
 
 // rho[x] = 1 - scale * sum_tiles part[tile][x] for x in [x0, x0 + XS)
 // (all FT threads; XS divides FT or FT divides XS)
-template <int FT, class Ctx>
+template <int FT, bool COH = false, class Ctx>
 VP_HD void field_density(Ctx& c, const FieldArgs& a, int x0, int XS, double* rho, double* tmp) {
   const int nx = a.nx;
   const int G = XS >= FT ? 1 : FT / XS;
   for (int l0 = c.tid; l0 < XS * G; l0 += FT) {
     const int xl = l0 % XS, g = l0 / XS;
     const double* p = a.part + x0 + xl;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    auto ld = [&](int tl) { return COH ? c.load_coherent(p + (size_t)tl * nx) : p[(size_t)tl * nx]; };
+    // eight independent partial sums keep eight loads in flight per thread
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int tile = g;
-    for (; tile + 3 * G < a.ntiles; tile += 4 * G) {
-      s0 += p[(size_t)tile * nx];
-      s1 += p[(size_t)(tile + G) * nx];
-      s2 += p[(size_t)(tile + 2 * G) * nx];
-      s3 += p[(size_t)(tile + 3 * G) * nx];
+    for (; tile + 7 * G < a.ntiles; tile += 8 * G) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] += ld(tile + u * G);
     }
-    for (; tile < a.ntiles; tile += G) s0 += p[(size_t)tile * nx];
-    tmp[l0] = (s0 + s1) + (s2 + s3);
+    for (int u = 0; tile < a.ntiles; tile += G, ++u) acc[u & 7] += ld(tile);
+    tmp[l0] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
   }
   c.sync();
   for (int xl = c.tid; xl < XS; xl += FT) {
@@ -384,7 +305,7 @@ VP_HD bool field_gather(Ctx& c, const FieldArgs& a, double* rho, double* tmp, in
   c.sync();
   if (!*flag) return false;
   c.fence();
-  for (int x = c.tid; x < nx; x += FT) rho[x] = c.load_coherent(a.rho_g + x);
+  for (int x = c.tid; x < nx; x += FT) rho[x] = a.rho_g[x];
   if (c.tid == 0) c.reset(a.counter);
   c.sync();
   return true;
@@ -411,7 +332,9 @@ struct FieldCfg {
   static constexpr int lds = tw_bytes + 2 * nx * 8 + work + 64 + 16;
 };
 
-template <int M, int Q, int FT, class Ctx>
+// TAIL: called by the last workgroup of the drift kernel, which reads every
+// partial density row itself (coherent loads) instead of the slice hand-off
+template <int M, int Q, int FT, bool TAIL = false, class Ctx>
 VP_HD void field_body(Ctx& c, unsigned char* lds, const FieldArgs& a) {
   using Cfg = FieldCfg<M, Q, FT>;
   constexpr int TPT = Cfg::TPT;
@@ -423,7 +346,12 @@ VP_HD void field_body(Ctx& c, unsigned char* lds, const FieldArgs& a) {
   double* tmp = reinterpret_cast<double*>(buf);
   double* rs = reinterpret_cast<double*>(lds + Cfg::lds - 80);
   int* flag = reinterpret_cast<int*>(lds + Cfg::lds - 16);
-  if (!field_gather<FT>(c, a, rho, tmp, flag)) return;
+  if constexpr (TAIL) {
+    // the caller's acquire fence makes the other workgroups' rows visible
+    field_density<FT, false>(c, a, 0, Cfg::nx, rho, tmp);
+  } else {
+    if (!field_gather<FT>(c, a, rho, tmp, flag)) return;
+  }
   tw_stage<double, Cfg::nx>(c, twl, a.tw);
   for (int x = c.tid; x < Cfg::nx; x += FT) okx[x] = a.okx[x];
   c.sync();
@@ -494,6 +422,125 @@ VP_HD void field_dft_body(Ctx& c, unsigned char* lds, const FieldArgs& a) {
     if (c.tid == 0) {
       a.series[0] = acc[0] / a.nx;
       a.series[1] = acc[1] / a.nx;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// drift kernel (x-advection of C columns) + partial density sums
+// ---------------------------------------------------------------------------
+template <class T>
+struct DriftArgs {
+  const T* f_in;
+  T* f_out;
+  int nx, nv;
+  double dt;
+  double dk;         // kx[1] of the fftfreq table
+  const double* v;   // [nv]
+  const cx<T>* tw;   // twiddle tables of exp(-2 pi i n / nx), [lo | hi]
+  double* dens;      // [gridDim][nx] partial sums (unscaled), or null
+  int fuse_field;    // 1: the last workgroup to finish solves the field (fld)
+  FieldArgs fld;     // field arguments for the fused tail (part = dens)
+  // batched scan: grid y = instance; f, dens, counter, okx, E and <E^2> are
+  // offset per instance; dk_inst[inst] replaces dk
+  const double* dk_inst;
+  int series_stride;  // doubles between consecutive instances' series entries
+};
+
+template <class T, int M, int Q, int C>
+struct DriftCfg {
+  static constexpr int TPT = M / Q;
+  static constexpr int wg = TPT * C;
+  static constexpr int stride = Padded<M>::size;
+  static constexpr int tw_bytes = (TwSize<2 * M>::total * (int)sizeof(cx<T>) + 15) & ~15;
+  static constexpr int own = tw_bytes + C * stride * (int)sizeof(cx<T>);
+  // fused field tail (small grids): M-point Poisson transform on the same workgroup
+  static constexpr int fq = M < 8 ? M : 8;
+  static constexpr bool tail = M <= 128 && (M / fq) <= wg;
+  static constexpr int field_lds = tail ? FieldCfg<M, fq, wg>::lds : 0;
+  static constexpr int lds = own > field_lds ? own : field_lds;
+};
+
+template <class T, int M, int Q, int C, class Ctx>
+VP_HD void drift_body(Ctx& c, unsigned char* lds, const DriftArgs<T>& a) {
+  using Cfg = DriftCfg<T, M, Q, C>;
+  constexpr int TPT = Cfg::TPT;
+  const int cc = c.tid % C;
+  const int t = c.tid / C;
+  cx<T>* twl = reinterpret_cast<cx<T>*>(lds);
+  cx<T>* buf = reinterpret_cast<cx<T>*>(lds + Cfg::tw_bytes) + cc * Cfg::stride;
+  const size_t nv = (size_t)a.nv;
+  const int inst = c.bidy;
+  const size_t foff = (size_t)inst * (size_t)a.nx * nv;
+  const double dk = a.dk_inst ? a.dk_inst[inst] : a.dk;
+  tw_stage<T, 2 * M>(c, twl, a.tw);
+  const Tw<T> tw{twl};
+  double acc[2 * Q];
+#pragma unroll
+  for (int i = 0; i < 2 * Q; ++i) acc[i] = 0.0;
+  const int ntiles = a.nv / C;
+  // the grid strides over the column tiles; workgroup b accumulates the
+  // density of every tile it advects into one partial row dens[b][x]
+  for (int tile = c.bid; tile < ntiles; tile += c.nblk) {
+    const int col = tile * C + cc;
+    const T* fi = a.f_in + foff + col;
+    cx<T> v[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int j = t + i * TPT;
+      v[i] = mk<T>(fi[(size_t)(2 * j) * nv], fi[(size_t)(2 * j + 1) * nv]);
+    }
+    const double vc = a.v[col];
+    c.sync();  // twiddle tables staged (first tile) / buffer free (later tiles)
+    spectral_apply<T, M, TPT>(c, v, buf, tw, t, PhaseMul<T>((dk * a.dt) * vc, M, t, TPT));
+    T* fo = a.f_out + foff + col;
+    const double w = trapw(col, a.nv);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const int j = t + i * TPT;
+      fo[(size_t)(2 * j) * nv] = v[i].re;
+      fo[(size_t)(2 * j + 1) * nv] = v[i].im;
+      acc[2 * i] += w * (double)v[i].re;
+      acc[2 * i + 1] += w * (double)v[i].im;
+    }
+  }
+  if (a.dens) {
+    double* out = a.dens + ((size_t)inst * c.nblk + c.bid) * a.nx;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      double s2[2] = {acc[2 * i], acc[2 * i + 1]};
+#pragma unroll
+      for (int m = 1; m < C; m <<= 1) {
+        s2[0] += c.shfl_xor(s2[0], m, C);
+        s2[1] += c.shfl_xor(s2[1], m, C);
+      }
+      if (cc == 0) {
+        const int j = t + i * TPT;
+        out[2 * j] = s2[0];
+        out[2 * j + 1] = s2[1];
+      }
+    }
+  }
+  if constexpr (Cfg::tail) {
+    if (a.fuse_field) {
+      // last workgroup to arrive reduces every partial row and solves Poisson
+      int* flag = reinterpret_cast<int*>(lds + Cfg::lds - 16);
+      FieldArgs fa = a.fld;
+      fa.part += (size_t)inst * c.nblk * a.nx;
+      fa.counter += inst * 16;
+      if (a.dk_inst) fa.okx += (size_t)inst * a.nx;
+      fa.e_out += (size_t)inst * a.nx;
+      if (fa.e_out2) fa.e_out2 += (size_t)inst * a.nx;
+      if (fa.driver) fa.driver += (size_t)inst * a.nx;
+      if (fa.series) fa.series += (size_t)inst * a.series_stride;
+      c.fence();
+      c.sync();
+      if (c.tid == 0) *flag = (c.arrive(fa.counter) == (unsigned)(c.nblk - 1)) ? 1 : 0;
+      c.sync();
+      if (!*flag) return;
+      c.fence();
+      field_body<M, Cfg::fq, Cfg::wg, true>(c, lds, fa);
+      if (c.tid == 0) c.reset(fa.counter);
     }
   }
 }

@@ -1,0 +1,139 @@
+"""Batched parameter scans: thousands of independent VPFP instances with
+uneven (nx, nv), wavenumber, amplitude and collision frequency advanced
+together on one GPU.
+
+Instances are grouped by size class (nx, nv); each class is one batch of the
+C ABI (vpfp_batch_*), whose f lives in HBM as [n_inst][nx][nv].  A step of a
+class is two launches (fused kick-collide-kick rows of every instance; drift
+of every instance's columns with each instance's field solved in the same
+launch).  Classes run concurrently on separate HIP streams.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .core import _dev
+
+
+def draw_parameters(n, seed, nx_choices=(32, 64, 128), nv_choices=(256, 512, 1024, 2048)):
+    """k ~ U[0.25, 0.45], amp = 10^U[-4, -1], nu = 10^U[-5, -2], nx and nv
+    uniform over the choices; drawn in that order from default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    k = rng.uniform(0.25, 0.45, n)
+    amp = 10.0 ** rng.uniform(-4.0, -1.0, n)
+    nu = 10.0 ** rng.uniform(-5.0, -2.0, n)
+    nx = rng.choice(np.asarray(nx_choices), n)
+    nv = rng.choice(np.asarray(nv_choices), n)
+    return {"k": k, "amp": amp, "nu": nu, "nx": nx, "nv": nv}
+
+
+def instance_grids(nx, k, nv, vmax):
+    """x, kx, 1/kx of a box [0, 2 pi/k) and v, kv, dv (initializers.py:56-89)."""
+    xmax = 2.0 * np.pi / k
+    dx = xmax / nx
+    x = np.linspace(dx / 2.0, xmax - dx / 2.0, nx)
+    kx = np.fft.fftfreq(nx, d=dx) * 2.0 * np.pi
+    okx = np.zeros_like(kx)
+    okx[1:] = 1.0 / kx[1:]
+    dv = 2 * vmax / nv
+    v = np.linspace(-vmax + dv / 2.0, vmax - dv / 2.0, nv)
+    kv = np.fft.fftfreq(nv, d=dv) * 2.0 * np.pi
+    return x, kx, okx, v, kv, dv
+
+
+class _Class:
+    """One size class (nx, nv): a C-ABI batch and its device state."""
+
+    def __init__(self, idx, params, vmax, dt, op, precision, dev):
+        self.idx = idx
+        self.n = idx.size
+        self.nx, self.nv = int(params["nx"][idx[0]]), int(params["nv"][idx[0]])
+        grids = [instance_grids(self.nx, params["k"][i], self.nv, vmax) for i in idx]
+        v, kv, dv = grids[0][3], grids[0][4], grids[0][5]
+        self.dv = dv
+        kx = np.ascontiguousarray(np.stack([g[1] for g in grids]))
+        okx = np.ascontiguousarray(np.stack([g[2] for g in grids]))
+        nu = np.ascontiguousarray(params["nu"][idx].astype(np.float64))
+        self._keep = (kx, okx, v, kv, nu)
+        P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+        desc = _lib.BatchDesc(self.n, self.nx, self.nv, _lib.VPFP_F64 if precision == "fp64" else _lib.VPFP_F32,
+                              P(kx), P(okx), P(v), P(kv), dv, P(nu), float(dt),
+                              _lib.VPFP_OP_DG if op == "dg" else _lib.VPFP_OP_LB)
+        h = ctypes.c_void_p()
+        _lib.check(_lib.library().vpfp_batch_create(ctypes.byref(desc), ctypes.byref(h)))
+        self.handle = h
+        # f0 = (1 + amp cos(k x)) exp(-v^2/2)/sqrt(2 pi), formed on the device
+        x = torch.as_tensor(np.stack([g[0] for g in grids]), device=dev)
+        k = torch.as_tensor(params["k"][idx], device=dev)[:, None]
+        amp = torch.as_tensor(params["amp"][idx], device=dev)[:, None]
+        dens = 1.0 + amp * torch.cos(k * x)
+        vt = torch.as_tensor(v, device=dev)
+        maxw = torch.exp(-vt ** 2 / 2.0) / np.sqrt(2.0 * np.pi)
+        self.f = (dens[:, :, None] * maxw[None, None, :]).to(_dev.f_dtype(precision)).contiguous()
+        self.e = torch.zeros(self.n, self.nx, dtype=torch.float64, device=dev)
+
+    def step(self, n_steps, series, stream):
+        _lib.check(_lib.library().vpfp_batch_step_n(
+            self.handle, ctypes.c_void_p(self.f.data_ptr()), ctypes.c_void_p(self.e.data_ptr()), int(n_steps),
+            ctypes.c_void_p(series.data_ptr()) if series is not None else None,
+            ctypes.c_void_p(stream.cuda_stream)))
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib._lib is not None:
+            _lib._lib.vpfp_batch_destroy(h)
+
+
+class Scan:
+    """A ragged parameter scan.
+
+    params: per-instance arrays k, amp, nu, nx, nv (see draw_parameters).
+    Each instance: x in [0, 2 pi/k), v in [-vmax, vmax], f0 = (1 + amp cos kx)
+    times a unit Maxwellian, e0 = 0, then every step the leapfrog split and
+    the implicit collision step (op "dg" or "lb").
+    """
+
+    def __init__(self, params, vmax=6.4, dt=0.1, op="dg", precision="fp32"):
+        self.params = {key: np.asarray(val) for key, val in params.items()}
+        self.n = self.params["k"].size
+        self.vmax, self.dt, self.precision = float(vmax), float(dt), precision
+        dev = _dev.device()
+        keys = sorted(set(zip(self.params["nx"].tolist(), self.params["nv"].tolist())))
+        self.classes = []
+        for nx, nv in keys:
+            idx = np.nonzero((self.params["nx"] == nx) & (self.params["nv"] == nv))[0]
+            self.classes.append(_Class(idx, self.params, vmax, dt, op, precision, dev))
+        self.streams = [torch.cuda.Stream() for _ in self.classes]
+        self.cells = int(np.sum(self.params["nx"].astype(np.int64) * self.params["nv"].astype(np.int64)))
+
+    def run(self, n_steps, record=True):
+        """Advance every instance n_steps.  Returns <E^2>(t) as [n_steps, n_inst]
+        (host) when record is true.  Work is enqueued on the classes' streams,
+        ordered after torch's current stream and joined back into it."""
+        dev = _dev.device()
+        series = [torch.zeros(n_steps, c.n, 2, dtype=torch.float64, device=dev) if record else None
+                  for c in self.classes]
+        cur = torch.cuda.current_stream()
+        for c, s, ser in zip(self.classes, self.streams, series):
+            s.wait_stream(cur)
+            c.step(n_steps, ser, s)
+        for s in self.streams:
+            cur.wait_stream(s)
+        if not record:
+            return None
+        out = np.zeros((n_steps, self.n))
+        for c, ser in zip(self.classes, series):
+            out[:, c.idx] = ser[:, :, 0].cpu().numpy()
+        return out
+
+    def density(self):
+        """Box-averaged trapz density of every instance (host)."""
+        out = np.zeros(self.n)
+        for c in self.classes:
+            f = c.f.double()
+            n = c.dv * (f.sum(-1) - 0.5 * (f[..., 0] + f[..., -1]))
+            out[c.idx] = n.mean(-1).cpu().numpy()
+        return out
