@@ -14,6 +14,13 @@ n, j, T, q, v^4, v^5, f^2, f log f per x, <E^2>).
 
 Prints one JSON line.  `value` is phase-space cell-updates per second
 (nx * nv * steps / wall time of the timed region, inputs resident in HBM).
+
+The line also carries the ragged parameter scan (`scan`): 4096 instances,
+k ~ U[0.25, 0.45], amplitude 10^U[-4, -1], nu 10^U[-5, -2] (Dougherty),
+nx in {32, 64, 128}, nv in {256, 512, 1024, 2048} drawn from
+default_rng(20261202); dt = 0.1, 2000 steps; batched FP32 advection with
+FP64 moments/tridiagonal against the same batch in FP64, instances/s and the
+per-instance damping-rate agreement of <E^2>.
 """
 
 import argparse
@@ -188,6 +195,134 @@ def roofline(run):
     }
 
 
+# --------------------------------------------------------------------------- scan
+
+SCAN_N, SCAN_SEED, SCAN_STEPS, SCAN_DT = 4096, 20261202, 2000, 0.1
+
+
+def scan_window(p, i, e2, t):
+    """Fit window for instance i: after one period, before trapping
+    (2 pi / sqrt(amp)), half the recurrence time 2 pi / (k dv) and the
+    point where <E^2> has fallen 1e8 below its start (FP32 noise)."""
+    import numpy as np
+
+    dv = 12.8 / p["nv"][i]
+    t_rec = 2 * np.pi / (p["k"][i] * dv)
+    t_trap = 2 * np.pi / np.sqrt(p["amp"][i])
+    y = e2[:, i]
+    floor = np.nonzero(y < 1e-8 * y[:20].max())[0]
+    t_noise = t[floor[0]] if floor.size else t[-1]
+    return 6.0, min(0.5 * t_trap, 0.5 * t_rec, t_noise, t[-1])
+
+
+def scan_run(precision, steps):
+    import torch
+
+    from vlapy_amd.scan import Scan, draw_parameters
+
+    p = draw_parameters(SCAN_N, SCAN_SEED)
+    sc = Scan(p, dt=SCAN_DT, op="dg", precision=precision)
+    sc.run(2, record=False)  # warmup (also first-call kernel setup)
+    torch.cuda.synchronize()
+    del sc
+    sc = Scan(p, dt=SCAN_DT, op="dg", precision=precision)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e2 = sc.run(steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return p, sc.cells, e2, wall
+
+
+def scan_cpu(p, steps_sample=10, n_sample=12):
+    """Oracle time per instance-step on a sample of instances (one thread),
+    extrapolated to the whole scan."""
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        env[k] = "1"
+    res = subprocess.run([sys.executable, os.path.abspath(__file__), "--scan-cpu-child",
+                          "%d,%d" % (steps_sample, n_sample)], capture_output=True, text=True, env=env, timeout=900)
+    if res.returncode != 0:
+        raise RuntimeError("scan cpu baseline failed: " + res.stderr[-2000:])
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
+def scan_cpu_child(arg):
+    import numpy as np
+
+    sys.path.insert(0, HERE)
+    from oracle import vpfp_oracle as O
+    from vlapy_amd.scan import draw_parameters, instance_grids
+
+    steps, n_sample = (int(a) for a in arg.split(","))
+    p = draw_parameters(SCAN_N, SCAN_SEED)
+    # a sample spread over the size classes: every (4096 / n_sample)-th instance
+    idx = np.arange(0, SCAN_N, SCAN_N // n_sample)[:n_sample]
+    per_cell = []
+    total = 0.0
+    for i in idx:
+        x, kx, okx, v, kv, dv = instance_grids(int(p["nx"][i]), p["k"][i], int(p["nv"][i]), 6.4)
+        g = O.Grid(x, kx, okx, v, kv, dv)
+        f = O.perturbed_maxwellian(x, v, p["k"][i], p["amp"][i])
+        t0 = time.perf_counter()
+        O.run_steps(g, np.zeros(x.size), f, steps, SCAN_DT, nu=p["nu"][i], fp_type="dg")
+        el = time.perf_counter() - t0
+        total += el
+        per_cell.append(el / steps / (x.size * v.size))
+    cells_all = np.sum(p["nx"].astype(float) * p["nv"].astype(float))
+    # seconds for the whole scan at the sample's mean time per cell-step
+    est = float(np.mean(per_cell)) * cells_all * SCAN_STEPS
+    print(json.dumps({"sample_instances": int(len(idx)), "sample_steps": steps, "sample_seconds": total,
+                      "est_scan_seconds": est}))
+
+
+def scan_section(steps, with_cpu):
+    import numpy as np
+
+    from vlapy_amd.diagnostics.low_level_helpers import get_standing_wave_rate_and_frequency as rate
+
+    p, cells, e2_32, wall32 = scan_run("fp32", steps)
+    _, _, e2_64, wall64 = scan_run("fp64", steps)
+    t = SCAN_DT * (np.arange(steps) + 1)
+    errs = []
+    for i in range(SCAN_N):
+        lo, hi = scan_window(p, i, e2_64, t)
+        try:
+            g64, _ = rate(e2_64[:, i], t, lo, hi)
+            g32, _ = rate(e2_32[:, i], t, lo, hi)
+        except ValueError:
+            continue
+        errs.append(abs(g32 / g64 - 1.0))
+    errs = np.asarray(errs)
+    out = {
+        "workload": "4096-instance ragged scan, seed 20261202, Dougherty, dt=0.1, %d steps" % steps,
+        "instances_per_s": SCAN_N / wall32,
+        "cell_updates_per_s": cells * steps / wall32,
+        "seconds": wall32,
+        "fp64_instances_per_s": SCAN_N / wall64,
+        "fp64_seconds": wall64,
+        "cells": cells,
+        "damping_rate": {
+            "instances_fitted": int(errs.size),
+            "within_1pct": float(np.mean(errs <= 0.01)) if errs.size else None,
+            "median_rel_err": float(np.median(errs)) if errs.size else None,
+            "p95_rel_err": float(np.percentile(errs, 95)) if errs.size else None,
+        },
+    }
+    if with_cpu:
+        c = scan_cpu(p)
+        est = c["est_scan_seconds"] * steps / SCAN_STEPS
+        out["cpu_baseline"] = {
+            "instances_per_s": SCAN_N / est,
+            "cores": 1,
+            "kind": "port",
+            "sample": "%d instances x %d steps of the scan on the NumPy FP64 oracle (%.1f s), extrapolated by "
+                      "cells and steps to the whole scan" % (c["sample_instances"], c["sample_steps"],
+                                                             c["sample_seconds"]),
+        }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -196,11 +331,17 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scan-steps", type=int, default=SCAN_STEPS)
+    ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--cpu-child", type=float, default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--scan-cpu-child", type=str, default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     if args.cpu_child is not None:
         cpu_child(args.cpu_child)
+        return 0
+    if args.scan_cpu_child is not None:
+        scan_cpu_child(args.scan_cpu_child)
         return 0
     if args.gpus != 1:
         print("bench.py measures one GPU only: the build has no multi-GPU path (instances of the "
@@ -246,6 +387,8 @@ def main():
         },
         "f32_vs_f64_after_run": {"E_max_rel": err_e, "f_max_rel": err_f},
     }
+    if not args.no_scan:
+        out["scan"] = scan_section(args.scan_steps, not args.no_cpu)
     print(json.dumps(out))
     return 0
 

@@ -176,23 +176,32 @@ VP_HD void dft(cx<T>* x) {
 }
 
 // ---------------------------------------------------------------------------
-// Twiddles W_N^i = exp(-2 pi i i / N) from two small tables staged in LDS:
-// W_N^i = hi[i >> 6] * lo[i & 63], hi[h] = W_N^{64 h}, lo[j] = W_N^j.
-// The global layout is [lo: 64 entries | hi: max(1, N/64) entries].
+// Twiddles W_N^i = exp(-2 pi i i / N) staged in LDS.  Large N: two small
+// tables, W_N^i = hi[i >> 6] * lo[i & 63], hi[h] = W_N^{64 h}, lo[j] = W_N^j,
+// global layout [lo: 64 entries | hi: max(1, N/64) entries].
 // ---------------------------------------------------------------------------
+// For N <= kTwFull the whole table W_N^i, i < N, is staged instead.
+constexpr int kTwFull = 1024;
+VP_HD constexpr bool tw_full(int N) { return N <= kTwFull; }
+VP_HD constexpr int tw_entries(int N) { return tw_full(N) ? N : 64 + (N / 64 > 1 ? N / 64 : 1); }
+
 template <int N>
 struct TwSize {
-  static constexpr int hi = N / 64 > 1 ? N / 64 : 1;
-  static constexpr int total = 64 + hi;
+  static constexpr bool full = tw_full(N);
+  static constexpr int total = tw_entries(N);
 };
 
-template <class T>
+template <class T, bool FULL = false>
 struct Tw {
-  const cx<T>* t;  // LDS copy, [lo | hi]
+  const cx<T>* t;  // LDS copy: W_N^i (FULL) or [lo | hi]
   VP_HD cx<T> operator()(int i) const {
-    const cx<T> lo = t[i & 63];
-    if (i < 64) return lo;
-    return mul(t[64 + (i >> 6)], lo);
+    if constexpr (FULL) {
+      return t[i];
+    } else {
+      const cx<T> lo = t[i & 63];
+      if (i < 64) return lo;
+      return mul(t[64 + (i >> 6)], lo);
+    }
   }
 };
 
@@ -215,8 +224,8 @@ VP_HD void tw_stage(Ctx& c, cx<T>* dst, const cx<T>* src) {
 // tw(n) = exp(-2 pi i n / (2M)); the M-point twiddle W_M^n is tw(2n).
 // ---------------------------------------------------------------------------
 // `act` false: the thread only takes part in the barriers (no LDS writes).
-template <class T, int M, int TPT, int R, int NS, bool INV, class Ctx>
-VP_HD void st_pass(Ctx& c, cx<T>* v, cx<T>* buf, const Tw<T>& tw, int t, bool act) {
+template <class T, int M, int TPT, int R, int NS, bool INV, class TW, class Ctx>
+VP_HD void st_pass(Ctx& c, cx<T>* v, cx<T>* buf, const TW& tw, int t, bool act) {
   constexpr int Q = M / TPT;
   constexpr int NB = Q / R;
   static_assert(NB * R == Q, "radix must divide the per-thread count");
@@ -247,8 +256,8 @@ VP_HD void st_pass(Ctx& c, cx<T>* v, cx<T>* buf, const Tw<T>& tw, int t, bool ac
   c.sync();
 }
 
-template <class T, int M, int TPT, bool INV, int NS = 1, class Ctx>
-VP_HD void fft(Ctx& c, cx<T>* v, cx<T>* buf, const Tw<T>& tw, int t, bool act = true) {
+template <class T, int M, int TPT, bool INV, int NS = 1, class TW, class Ctx>
+VP_HD void fft(Ctx& c, cx<T>* v, cx<T>* buf, const TW& tw, int t, bool act = true) {
   constexpr int Q = M / TPT;
   static_assert(Q >= 2 && Q * TPT == M, "each thread must hold at least two values");
   constexpr int RMAX = Q < 16 ? Q : 16;
@@ -315,8 +324,8 @@ struct PoissonMul {
 // the repacked spectrum Z'_k = (F'_k + F'_{k+M} + i (F'_k - F'_{k+M}) conj W)/N
 // inverse-transforms (unnormalised M-point) to f'_{2j} + i f'_{2j+1}.
 // v[i] holds Z_k and pz[i] holds Z_{M-k} for k = t + i*TPT; tw(k) = W^k.
-template <class T, int M, int TPT, class Mul>
-VP_HD void couple(cx<T>* v, const cx<T>* pz, const Tw<T>& tw, int t, Mul& mult) {
+template <class T, int M, int TPT, class Mul, class TW>
+VP_HD void couple(cx<T>* v, const cx<T>* pz, const TW& tw, int t, Mul& mult) {
   constexpr int Q = M / TPT;
   const T invN = T(1) / T(2 * M);
   const T h = T(0.5);
@@ -384,29 +393,33 @@ template <int TPT, int L, class Coef, class Ctx>
 VP_HD void tridiag(Ctx& c, double* d, const Coef& cf, int t, double* red) {
   static_assert(L >= 2, "chunk must hold at least two rows");
   const int m0 = t * L;
-  // sweep down keeping x_{m0} free: x_i = p + q x_{m0} - r x_{i+1}
-  double p = 0, q = 0, r = 0;
+  // Two independent sweeps, interleaved so that their dependency chains
+  // overlap:
+  //   down (i = 1 .. L-1), x_{m0} kept free:   x_i = p  + q  x_{m0}     - r  x_{i+1}
+  //   up   (i = L-2 .. 0), x_{m0+L-1} kept free: x_i = p2 + q2 x_{m0+L-1} - r2 x_{i-1}
+  double p = 0, q = 0, r = 0, p2 = 0, q2 = 0, r2 = 0;
 #pragma unroll
-  for (int i = 1; i < L; ++i) {
-    const int m = m0 + i;
-    const double lm = cf.l(m), um = cf.u(m), bm = cf.b(m);
-    const double den = (i == 1) ? bm : bm - lm * r;
-    const double inv = rcp(den);
-    p = (i == 1) ? d[i] * inv : (d[i] - lm * p) * inv;
-    q = (i == 1) ? -lm * inv : -lm * q * inv;
-    r = um * inv;
-  }
-  // sweep up keeping x_{m0+L-1} free: x_i = p2 + q2 x_{m0+L-1} - r2 x_{i-1}
-  double p2 = 0, q2 = 0, r2 = 0;
-#pragma unroll
-  for (int i = L - 2; i >= 0; --i) {
-    const int m = m0 + i;
-    const double lm = cf.l(m), um = cf.u(m), bm = cf.b(m);
-    const double den = (i == L - 2) ? bm : bm - um * r2;
-    const double inv = rcp(den);
-    p2 = (i == L - 2) ? d[i] * inv : (d[i] - um * p2) * inv;
-    q2 = (i == L - 2) ? -um * inv : -um * q2 * inv;
-    r2 = lm * inv;
+  for (int s = 0; s < L - 1; ++s) {
+    {
+      const int i = 1 + s;
+      const int m = m0 + i;
+      const double lm = cf.l(m), um = cf.u(m), bm = cf.b(m);
+      const double den = (i == 1) ? bm : bm - lm * r;
+      const double inv = rcp(den);
+      p = (i == 1) ? d[i] * inv : (d[i] - lm * p) * inv;
+      q = (i == 1) ? -lm * inv : -lm * q * inv;
+      r = um * inv;
+    }
+    {
+      const int i = L - 2 - s;
+      const int m = m0 + i;
+      const double lm = cf.l(m), um = cf.u(m), bm = cf.b(m);
+      const double den = (i == L - 2) ? bm : bm - um * r2;
+      const double inv = rcp(den);
+      p2 = (i == L - 2) ? d[i] * inv : (d[i] - um * p2) * inv;
+      q2 = (i == L - 2) ? -um * inv : -um * q2 * inv;
+      r2 = lm * inv;
+    }
   }
   // interface system: y_{2t} = x_{m0}, y_{2t+1} = x_{m0+L-1}
   //   row 2t  :  r2 y_{2t-1} + y_{2t} - q2 y_{2t+1} = p2
