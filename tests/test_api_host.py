@@ -99,6 +99,8 @@ def test_diagnostics_match_oracle():
     np.testing.assert_allclose(g1, g2, rtol=1e-12)
     assert g1[0] == pytest.approx(-0.0126, rel=2e-3) and g1[1] == pytest.approx(1.16, rel=2e-3)
     assert llh.get_damping_rate(e, t=t) == pytest.approx(O.damping_rate_reference(e, t), rel=1e-12)
+    with pytest.raises(ValueError):
+        llh.get_standing_wave_rate_and_frequency(np.mean(e ** 2, axis=1), t, t_min=6.0, t_max=3.0)
 
 
 def test_post_inner_loop_update_matches_oracle():

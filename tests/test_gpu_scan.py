@@ -3,7 +3,9 @@
 Each instance of a small ragged scan (several size classes, per-instance k,
 amplitude and nu) must match the oracle's single-instance leapfrog +
 collision loop (FP64 batch: 1e-11 in the <E^2> history and f after 20 steps)
-and the FP32 batch must stay within its stated error.
+and the FP32 batch must stay within its stated error: f is stored in FP32 and
+each step makes four FP32 transform passes of <= 10 radix levels, so the
+bound on f is 20 steps x 4 passes x ~4 ulp(f_max) ~ 2e-5 relative.
 """
 
 import numpy as np
@@ -50,7 +52,7 @@ def test_scan_matches_single_instance_oracle(precision, tol):
         c = [c for c in sc.classes if i in c.idx][0]
         j = int(np.nonzero(c.idx == i)[0][0])
         f_g = c.f[j].double().cpu().numpy()
-        assert np.abs(f_g - f_o).max() / np.abs(f_o).max() < (1e-12 if precision == "fp64" else 2e-6)
+        assert np.abs(f_g - f_o).max() / np.abs(f_o).max() < (1e-12 if precision == "fp64" else 2e-5)
 
 
 def test_scan_c3_draw_runs_and_conserves_density():

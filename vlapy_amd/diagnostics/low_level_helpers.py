@@ -68,6 +68,8 @@ def get_standing_wave_rate_and_frequency(mean_e2, t, t_min=None, t_max=None):
     if t_max is not None:
         keep &= t <= t_max
     idx = np.nonzero(keep)[0]
+    if idx.size < 3:
+        raise ValueError("the window [t_min, t_max] holds fewer than three samples")
     tp, lp = [], []
     for i in range(max(idx[0], 1), min(idx[-1], y.size - 2) + 1):
         if y[i] > y[i - 1] and y[i] >= y[i + 1]:
