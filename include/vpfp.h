@@ -150,6 +150,33 @@ int vpfp_batch_create(const vpfp_batch_desc* desc, vpfp_batch_t* out);
 int vpfp_batch_step_n(vpfp_batch_t batch, void* f, double* e, int n_steps, double* series, void* stream);
 int vpfp_batch_destroy(vpfp_batch_t batch);
 
+/* Launch shapes of the step and the autotuner that picks them.  No
+ * counterpart in the reference (its NumPy path has no launch shapes); the
+ * choices change rounding only, never the algorithm.
+ *   row_q       values per thread of the v-direction row kernels: 4 or 8
+ *               (0 = by row count: 4 when the grid has few rows)
+ *   drift_tile  x-advection columns per workgroup: 4, 8 or 16 (0 = keep)
+ *   fuse_field  1: the last drift workgroup solves the field (nx <= 256);
+ *               0 or -1 (default): separate field launch.  Batches always
+ *               use the separate launch and report 0.
+ *   us_per_step measured time of the chosen shape (autotune; else 0)
+ * vpfp_plan_autotune / vpfp_batch_autotune time n_steps steps of every valid
+ * shape on scratch state the call allocates and frees (the caller's f and E
+ * are not touched), keep the fastest, and synchronise the stream. */
+typedef struct vpfp_tuning {
+  int row_q;
+  int drift_tile;
+  int fuse_field;
+  double us_per_step;
+} vpfp_tuning;
+
+int vpfp_plan_get_tuning(vpfp_plan_t plan, vpfp_tuning* out);
+int vpfp_plan_set_tuning(vpfp_plan_t plan, const vpfp_tuning* tuning);
+int vpfp_plan_autotune(vpfp_plan_t plan, int n_steps, void* stream, vpfp_tuning* chosen /* or NULL */);
+int vpfp_batch_get_tuning(vpfp_batch_t batch, vpfp_tuning* out);
+int vpfp_batch_set_tuning(vpfp_batch_t batch, const vpfp_tuning* tuning);
+int vpfp_batch_autotune(vpfp_batch_t batch, int n_steps, void* stream, vpfp_tuning* chosen /* or NULL */);
+
 /* Instrumentation (no counterpart in the reference, which only logs wall
  * time per step to mlflow, manager.py:105-172).  With profiling on,
  * vpfp_step_n brackets each kernel launch with HIP events on its stream and

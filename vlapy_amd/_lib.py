@@ -57,6 +57,19 @@ class BatchDesc(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("row_q", _c_i),
+        ("drift_tile", _c_i),
+        ("fuse_field", _c_i),
+        ("us_per_step", _c_d),
+    ]
+
+    def as_dict(self):
+        return {"row_q": self.row_q, "drift_tile": self.drift_tile, "fuse_field": self.fuse_field,
+                "us_per_step": self.us_per_step}
+
+
 # name -> (argtypes); every function returns int status unless listed in _OTHER
 SIGNATURES = {
     "vpfp_plan_create": [ctypes.POINTER(GridDesc), ctypes.POINTER(_c_p)],
@@ -77,6 +90,12 @@ SIGNATURES = {
     "vpfp_batch_destroy": [_c_p],
     "vpfp_plan_set_profiling": [_c_p, _c_i],
     "vpfp_plan_kernel_times": [_c_p, ctypes.POINTER(_c_d), ctypes.POINTER(ctypes.c_longlong)],
+    "vpfp_plan_get_tuning": [_c_p, ctypes.POINTER(Tuning)],
+    "vpfp_plan_set_tuning": [_c_p, ctypes.POINTER(Tuning)],
+    "vpfp_plan_autotune": [_c_p, _c_i, _c_p, ctypes.POINTER(Tuning)],
+    "vpfp_batch_get_tuning": [_c_p, ctypes.POINTER(Tuning)],
+    "vpfp_batch_set_tuning": [_c_p, ctypes.POINTER(Tuning)],
+    "vpfp_batch_autotune": [_c_p, _c_i, _c_p, ctypes.POINTER(Tuning)],
     "vpfp_last_error": [],
     "vpfp_version": [],
 }
@@ -159,6 +178,30 @@ class Plan:
         n = (ctypes.c_longlong * 4)()
         check(library().vpfp_plan_kernel_times(self.handle, ms, n))
         return {k: (ms[i], n[i]) for i, k in enumerate(("row", "drift", "field", "other"))}
+
+    def tuning(self):
+        """Launch shapes in use: row_q, drift_tile, fuse_field, us_per_step."""
+        t = Tuning()
+        check(library().vpfp_plan_get_tuning(self.handle, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, row_q=None, drift_tile=None, fuse_field=None):
+        t = Tuning()
+        check(library().vpfp_plan_get_tuning(self.handle, ctypes.byref(t)))
+        if row_q is not None:
+            t.row_q = int(row_q)
+        if drift_tile is not None:
+            t.drift_tile = int(drift_tile)
+        if fuse_field is not None:
+            t.fuse_field = int(fuse_field)
+        check(library().vpfp_plan_set_tuning(self.handle, ctypes.byref(t)))
+
+    def autotune(self, n_steps=20, stream=None):
+        """Time every valid launch shape of the step (on scratch state) and
+        keep the fastest; returns the chosen tuning."""
+        t = Tuning()
+        check(library().vpfp_plan_autotune(self.handle, int(n_steps), stream, ctypes.byref(t)))
+        return t.as_dict()
 
     def __del__(self):
         h = getattr(self, "handle", None)

@@ -66,11 +66,13 @@ __global__ __launch_bounds__((vp::DriftCfg<T, M, Q, C>::wg)) void k_drift(vp::Dr
 
 constexpr int kFT = 256;
 
+// grid (nblk, 1): one field solve over nblk workgroups; grid (1, n): the n
+// instances of a batch, one workgroup each
 template <int M, int Q>
 __global__ __launch_bounds__(kFT) void k_field(vp::FieldArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   VPFP_CTX;
-  vp::field_body<M, Q, kFT>(c, lds, a);
+  vp::field_body<M, Q, kFT>(c, lds, vp::field_instance(a, c.bidy));
 }
 
 __global__ __launch_bounds__(kFT) void k_field_dft(vp::FieldArgs a) {
@@ -124,6 +126,9 @@ int set_lds(K kern, int bytes) {
 
 constexpr int qfor(int M) { return M < 8 ? M : 8; }
 
+// LDS per workgroup on gfx950
+constexpr int kMaxLds = 160 * 1024;
+
 // drift workgroups: one per column tile up to 1024, then grid-stride
 constexpr int kDriftMaxWG = 1024;
 int drift_grid(int ntiles) { return ntiles < kDriftMaxWG ? ntiles : kDriftMaxWG; }
@@ -144,10 +149,12 @@ int launch_row_q(const vp::RowArgs<T>& a, hipStream_t s) {
 // Q = 8 values per thread fills the GPU when there are many rows; with few
 // rows (fewer than ~16k threads at Q = 8) Q = 4 spreads each row over twice
 // as many threads and shortens the per-row latency chain.
+// q: 4 or 8 forces that many values per thread (where both exist), 0 = the
+// row-count rule above
 template <class T, int M>
-int launch_row(const vp::RowArgs<T>& a, hipStream_t s) {
+int launch_row(const vp::RowArgs<T>& a, hipStream_t s, int q) {
   if constexpr (M >= 16 && M <= 2048) {
-    if ((long long)a.nx * (M / 8) < 16384) return launch_row_q<T, M, 4>(a, s);
+    if (q == 4 || (q == 0 && (long long)a.nx * (M / 8) < 16384)) return launch_row_q<T, M, 4>(a, s);
   }
   return launch_row_q<T, M, qfor(M)>(a, s);
 }
@@ -168,9 +175,14 @@ int launch_drift_c(const vp::DriftArgs<T>& a, hipStream_t s, int gy) {
   constexpr int Q = M <= 64 ? (M < 4 ? M : 4) : 8;
   using Cfg = vp::DriftCfg<T, M, Q, C>;
   auto kern = k_drift<T, M, Q, C>;
+  if (Cfg::wg > 1024 || Cfg::lds > kMaxLds)
+    return fail(VPFP_EUNSUPPORTED, "drift tile of " + std::to_string(C) + " columns does not fit a workgroup");
   if (a.fuse_field && !Cfg::tail) return fail(VPFP_EUNSUPPORTED, "field tail not available at this size");
   if (int rc = set_lds(kern, Cfg::lds)) return rc;
-  hipLaunchKernelGGL(kern, dim3(gy > 1 ? a.nv / C : drift_grid(a.nv / C), gy), dim3(Cfg::wg), Cfg::lds, s, a);
+  // gy > 0: batch of gy instances, one column tile per workgroup; gy == 0:
+  // one grid, at most drift_grid(ntiles) workgroups striding over the tiles
+  const dim3 grid = gy > 0 ? dim3(a.nv / C, gy) : dim3(drift_grid(a.nv / C));
+  hipLaunchKernelGGL(kern, grid, dim3(Cfg::wg), Cfg::lds, s, a);
   return launched();
 }
 
@@ -185,12 +197,13 @@ int launch_drift(const vp::DriftArgs<T>& a, int C, hipStream_t s, int gy) {
 }
 
 template <int M>
-int launch_field(const vp::FieldArgs& a, hipStream_t s) {
+int launch_field(const vp::FieldArgs& a, hipStream_t s, int n_inst) {
   constexpr int Q = (M / kFT) > 8 ? (M / kFT) : (M < 8 ? M : 8);
   using Cfg = vp::FieldCfg<M, Q, kFT>;
   auto kern = k_field<M, Q>;
   if (int rc = set_lds(kern, Cfg::lds)) return rc;
-  hipLaunchKernelGGL(kern, dim3(field_grid(a.nx)), dim3(kFT), Cfg::lds, s, a);
+  const dim3 grid = n_inst > 0 ? dim3(1, n_inst) : dim3(field_grid(a.nx));
+  hipLaunchKernelGGL(kern, grid, dim3(kFT), Cfg::lds, s, a);
   return launched();
 }
 
@@ -214,9 +227,9 @@ int launch_field(const vp::FieldArgs& a, hipStream_t s) {
   }
 
 template <class T>
-int row_dispatch(const vp::RowArgs<T>& a, hipStream_t s) {
+int row_dispatch(const vp::RowArgs<T>& a, hipStream_t s, int q = 0) {
   const int M = a.nv / 2;
-#define CALL(m) launch_row<T, ((m) <= 4096 ? (m) : 2)>(a, s)
+#define CALL(m) launch_row<T, ((m) <= 4096 ? (m) : 2)>(a, s, q)
   VPFP_SWITCH_M(M, CALL, 4096)
 #undef CALL
   return fail(VPFP_EUNSUPPORTED, "nv must be a power of two in [4, 8192]");
@@ -232,7 +245,7 @@ int tri_dispatch(const vp::TriArgs<T>& a, hipStream_t s) {
 }
 
 template <class T>
-int drift_dispatch(const vp::DriftArgs<T>& a, int C, hipStream_t s, int gy = 1) {
+int drift_dispatch(const vp::DriftArgs<T>& a, int C, hipStream_t s, int gy = 0) {
   const int M = a.nx / 2;
 #define CALL(m) launch_drift<T, ((m) <= 1024 ? (m) : 2)>(a, C, s, gy)
   VPFP_SWITCH_M(M, CALL, 1024)
@@ -240,13 +253,15 @@ int drift_dispatch(const vp::DriftArgs<T>& a, int C, hipStream_t s, int gy = 1) 
   return fail(VPFP_EUNSUPPORTED, "nx must be a power of two in [4, 2048] for the x-advection");
 }
 
-int field_dispatch(const vp::FieldArgs& a, bool pow2, hipStream_t s) {
+// n_inst > 0: a batch of n_inst instance-major field solves (pow2 only)
+int field_dispatch(const vp::FieldArgs& a, bool pow2, hipStream_t s, int n_inst = 0) {
   if (pow2) {
     const int M = a.nx / 2;
-#define CALL(m) launch_field<m>(a, s)
+#define CALL(m) launch_field<m>(a, s, n_inst)
     VPFP_SWITCH_M(M, CALL, 2048)
 #undef CALL
   }
+  if (n_inst > 0) return fail(VPFP_EUNSUPPORTED, "batched field solve needs nx a power of two");
   if (a.nx > 4096) return fail(VPFP_EUNSUPPORTED, "nx must be at most 4096 for the field solve");
   const int lds = ((a.nx * 8 + 15) & ~15) + (a.nx > kFT ? a.nx : kFT) * 16 + 64;
   if (int rc = set_lds(k_field_dft, lds)) return rc;
@@ -291,6 +306,9 @@ struct vpfp_plan_s {
   double dk_x = 0, dk_v = 0, v0 = 0, dvv = 0;
   bool nx_pow2 = false, nv_pow2 = false;
   int C = 0, ntiles = 0;
+  int row_q = 0;   // values per thread of the row kernels (0: row-count rule)
+  int fuse = -1;   // field solved by the last drift workgroup: 1 yes, 0 no, -1 size rule
+  double tuned_us = 0.0;
   double *kx = nullptr, *okx = nullptr, *v = nullptr, *kv = nullptr;
   void* tw_v = nullptr;          // cx<T> [lo | hi] tables for nv
   void* tw_x = nullptr;          // cx<T> [lo | hi] tables for nx
@@ -386,6 +404,14 @@ int drift_tile(int nx, int nv) {
   return C >= 4 ? C : 0;
 }
 
+// Field solve of the fused step: as a tail of the drift (its last workgroup)
+// or as a launch of its own.  The tail costs every drift workgroup a
+// device-scope release fence, which on this multi-XCD part writes back and
+// invalidates the L2 (buffer_wbl2 / buffer_inv); measured at nx = 64,
+// nv = 1024 the separate launch is faster (23.9 vs 26.0 us per step), so the
+// tail is left to the autotuner.
+bool default_fuse(int) { return false; }
+
 hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 enum KernelKind { KK_ROW = 0, KK_DRIFT = 1, KK_FIELD = 2, KK_OTHER = 3 };
@@ -463,7 +489,6 @@ vp::DriftArgs<T> drift_args(vpfp_plan_t p, const void* fin, void* fout, double d
   a.fuse_field = 0;
   std::memset(&a.fld, 0, sizeof(a.fld));
   a.dk_inst = nullptr;
-  a.series_stride = 0;
   return a;
 }
 
@@ -482,6 +507,7 @@ vp::FieldArgs field_args(vpfp_plan_t p, const double* part, int ntiles, double s
   a.e_out = e_out;
   a.e_out2 = e_out2;
   a.series = series;
+  a.series_stride = 0;
   return a;
 }
 
@@ -510,7 +536,7 @@ int do_edfdv(vpfp_plan_t p, const void* fin, void* fout, const double* e, double
   a.e = e;
   a.dt_a = dt;
   a.flags = vp::RK_KICK_A;
-  return row_dispatch<T>(a, s);
+  return row_dispatch<T>(a, s, p->row_q);
 }
 
 template <class T>
@@ -530,7 +556,7 @@ template <class T>
 int do_collide(vpfp_plan_t p, const void* fin, void* fout, hipStream_t s) {
   auto a = row_args<T>(p, fin, fout);
   a.flags = vp::RK_COLLIDE;
-  return row_dispatch<T>(a, s);
+  return row_dispatch<T>(a, s, p->row_q);
 }
 
 template <class T>
@@ -565,10 +591,9 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
   ra.e = e;
   ra.dt_a = half;
   ra.flags = vp::RK_KICK_A;
-  if (n > 0) PROF(KK_ROW, row_dispatch<T>(ra, s));
+  if (n > 0) PROF(KK_ROW, row_dispatch<T>(ra, s, p->row_q));
   const double* e_cur = e;
-  // small grids: the last drift workgroup solves the field (one launch less)
-  const bool fuse = p->nx <= 256;
+  const bool fuse = p->fuse >= 0 ? p->fuse == 1 : default_fuse(p->nx);
   for (int i = 0; i < n; ++i) {
     double* ei = e_hist ? e_hist + (size_t)i * p->nx : p->ering + (size_t)(i & 1) * p->nx;
     auto fa = field_args(p, p->dens, p->ntiles, p->dv, driver ? driver + (size_t)i * p->nx : nullptr, ei, nullptr,
@@ -590,7 +615,7 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
     rb.moments = moments ? moments + (size_t)i * vp::kMoments * p->nx : nullptr;
     rb.flags = vp::RK_KICK_A | (coll ? vp::RK_COLLIDE : 0) | (moments ? vp::RK_MOMENTS : 0) |
                (i + 1 < n ? vp::RK_KICK_B : 0);
-    PROF(KK_ROW, row_dispatch<T>(rb, s));
+    PROF(KK_ROW, row_dispatch<T>(rb, s, p->row_q));
   }
   if (n > 0) VPFP_HIP(hipMemcpyAsync(e, e_cur, sizeof(double) * p->nx, hipMemcpyDeviceToDevice, s));
   return prof_collect(p);
@@ -601,6 +626,8 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
 // ====================================================================== batched scan
 struct vpfp_batch_s {
   int n = 0, nx = 0, nv = 0, prec = VPFP_F64, op = VPFP_OP_DG, C = 0, gx = 0;
+  int row_q = 0;
+  double tuned_us = 0.0;
   double dv = 0, dt = 0, v0 = 0, dvv = 0, dk_v = 0;
   double *dk_x = nullptr, *okx = nullptr, *nu_dt = nullptr, *v = nullptr;
   void *tw_v = nullptr, *tw_x = nullptr;
@@ -664,7 +691,7 @@ int batch_step_n(vpfp_batch_s* b, void* f, double* e, int n, double* series, hip
   ra.e = e;
   ra.flags = vp::RK_KICK_A;
   if (n > 0)
-    if (int rc = row_dispatch<T>(ra, s)) return rc;
+    if (int rc = row_dispatch<T>(ra, s, b->row_q)) return rc;
   const double* e_cur = e;
   for (int i = 0; i < n; ++i) {
     double* ei = b->ering + (size_t)(i & 1) * b->n * b->nx;
@@ -679,24 +706,178 @@ int batch_step_n(vpfp_batch_s* b, void* f, double* e, int n, double* series, hip
     da.tw = reinterpret_cast<const cx<T>*>(b->tw_x);
     da.dens = b->dens;
     da.dk_inst = b->dk_x;
-    da.fuse_field = 1;
-    da.series_stride = 2;
-    da.fld.part = b->dens;
-    da.fld.ntiles = b->gx;
-    da.fld.nx = b->nx;
-    da.fld.scale = b->dv;
-    da.fld.counter = b->counters;
-    da.fld.okx = b->okx;
-    da.fld.tw = b->tw_xs;
-    da.fld.e_out = ei;
-    da.fld.series = series ? series + (size_t)i * b->n * 2 : nullptr;
+    // the field is a launch of its own: a last-workgroup hand-off inside the
+    // drift would need a device-scope fence (an L2 write-back and invalidate)
+    // in each of the n * gx drift workgroups
+    vp::FieldArgs fa;
+    std::memset(&fa, 0, sizeof(fa));
+    fa.part = b->dens;
+    fa.ntiles = b->gx;
+    fa.nx = b->nx;
+    fa.scale = b->dv;
+    fa.counter = b->counters;
+    fa.okx = b->okx;
+    fa.tw = b->tw_xs;
+    fa.e_out = ei;
+    fa.series = series ? series + (size_t)i * b->n * 2 : nullptr;
+    fa.series_stride = 2;
     if (int rc = drift_dispatch<T>(da, b->C, s, b->n)) return rc;
+    if (int rc = field_dispatch(fa, true, s, b->n)) return rc;
     e_cur = ei;
     ra.e = e_cur;
     ra.flags = vp::RK_KICK_A | vp::RK_COLLIDE | (i + 1 < n ? vp::RK_KICK_B : 0);
-    if (int rc = row_dispatch<T>(ra, s)) return rc;
+    if (int rc = row_dispatch<T>(ra, s, b->row_q)) return rc;
   }
   if (n > 0) VPFP_HIP(hipMemcpyAsync(e, e_cur, sizeof(double) * b->n * b->nx, hipMemcpyDeviceToDevice, s));
+  return VPFP_OK;
+}
+
+// ---------------------------------------------------------------- autotuner
+// Times every valid launch shape of the step on scratch state and keeps the
+// fastest: values per thread of the row kernels (4 or 8, where both exist),
+// columns per drift workgroup (4, 8, 16) and, for a single grid with
+// nx <= 256, the field solve as a drift tail or as its own launch.
+
+// (1 + 0.01 cos(2 pi x / nx)) exp(-v^2 / 2) / sqrt(2 pi) on every row
+template <class T>
+std::vector<T> tune_state(size_t rows, int nx, int nv, double v0, double dvv) {
+  std::vector<T> h(rows * (size_t)nv);
+  const double two_pi = 6.283185307179586;
+  for (size_t r = 0; r < rows; ++r) {
+    const double n = 1.0 + 0.01 * std::cos(two_pi * (double)(r % nx) / nx);
+    for (int m = 0; m < nv; ++m) {
+      const double vm = v0 + m * dvv;
+      h[r * nv + m] = (T)(n * std::exp(-0.5 * vm * vm) / std::sqrt(two_pi));
+    }
+  }
+  return h;
+}
+
+struct TuneScratch {
+  void* f = nullptr;
+  double* e = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  ~TuneScratch() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (f) (void)hipFree(f);
+    if (e) (void)hipFree(e);
+  }
+  template <class T>
+  int init(const std::vector<T>& h, size_t ne) {
+    VPFP_HIP(hipMalloc(&f, sizeof(T) * h.size()));
+    VPFP_HIP(hipMalloc((void**)&e, sizeof(double) * ne));
+    VPFP_HIP(hipMemcpy(f, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    VPFP_HIP(hipMemset(e, 0, sizeof(double) * ne));
+    VPFP_HIP(hipEventCreate(&a));
+    VPFP_HIP(hipEventCreate(&b));
+    return VPFP_OK;
+  }
+  // microseconds per step of run(n_steps) after run(2) as warm-up
+  template <class Run>
+  int time(Run run, int n_steps, hipStream_t s, double* us) {
+    if (int rc = run(2)) return rc;
+    VPFP_HIP(hipEventRecord(a, s));
+    if (int rc = run(n_steps)) return rc;
+    VPFP_HIP(hipEventRecord(b, s));
+    VPFP_HIP(hipEventSynchronize(b));
+    float ms = 0.0f;
+    VPFP_HIP(hipEventElapsedTime(&ms, a, b));
+    *us = 1e3 * (double)ms / n_steps;
+    return VPFP_OK;
+  }
+};
+
+std::vector<int> row_q_candidates(int nv) {
+  const int M = nv / 2;
+  if (M >= 16 && M <= 2048) return {4, 8};
+  return {8};
+}
+
+template <class T>
+int autotune_plan(vpfp_plan_t p, int n_steps, hipStream_t s) {
+  TuneScratch w;
+  if (int rc = w.init(tune_state<T>(p->nx, p->nx, p->nv, p->v0, p->dvv), 2 * (size_t)p->nx)) return rc;
+  const int q0 = p->row_q, c0 = p->C, f0 = p->fuse;
+  const bool prof = p->profiling;
+  p->profiling = false;
+  double best = 1e300;
+  int bq = q0, bc = c0, bf = f0, rc = VPFP_OK;
+  for (int q : row_q_candidates(p->nv)) {
+    for (int C : {4, 8, 16}) {
+      if (C > p->nv) continue;
+      for (int fuse : {0, 1}) {
+        if (fuse && p->nx > 256) continue;
+        p->row_q = q;
+        p->C = C;
+        p->ntiles = drift_grid(p->nv / C);
+        p->fuse = fuse;
+        double us = 0.0;
+        rc = w.time([&](int n) { return do_step_n<T>(p, w.f, w.e, n, nullptr, nullptr, nullptr, nullptr, s); },
+                    n_steps, s, &us);
+        if (rc == VPFP_EUNSUPPORTED) continue;  // shape does not fit this grid
+        if (rc) break;
+        if (us < best) {
+          best = us;
+          bq = q, bc = C, bf = fuse;
+        }
+      }
+      if (rc && rc != VPFP_EUNSUPPORTED) break;
+    }
+    if (rc && rc != VPFP_EUNSUPPORTED) break;
+  }
+  p->profiling = prof;
+  const bool ok = !(rc && rc != VPFP_EUNSUPPORTED) && best < 1e300;
+  p->row_q = ok ? bq : q0;
+  p->C = ok ? bc : c0;
+  p->ntiles = drift_grid(p->nv / p->C);
+  p->fuse = ok ? bf : f0;
+  if (!ok) return rc ? rc : fail(VPFP_EHIP, "autotune: no launch shape ran");
+  p->tuned_us = best;
+  return VPFP_OK;
+}
+
+template <class T>
+int autotune_batch(vpfp_batch_s* b, int n_steps, hipStream_t s) {
+  TuneScratch w;
+  const size_t rows = (size_t)b->n * b->nx;
+  if (int rc = w.init(tune_state<T>(rows, b->nx, b->nv, b->v0, b->dvv), rows)) return rc;
+  const int q0 = b->row_q, c0 = b->C;
+  double best = 1e300;
+  int bq = q0, bc = c0, rc = VPFP_OK;
+  for (int q : row_q_candidates(b->nv)) {
+    for (int C : {4, 8, 16}) {
+      if (C > b->nv) continue;
+      b->row_q = q;
+      b->C = C;
+      b->gx = b->nv / C;
+      double us = 0.0;
+      rc = w.time([&](int n) { return batch_step_n<T>(b, w.f, w.e, n, nullptr, s); }, n_steps, s, &us);
+      if (rc == VPFP_EUNSUPPORTED) continue;
+      if (rc) break;
+      if (us < best) {
+        best = us;
+        bq = q, bc = C;
+      }
+    }
+    if (rc && rc != VPFP_EUNSUPPORTED) break;
+  }
+  const bool ok = !(rc && rc != VPFP_EUNSUPPORTED) && best < 1e300;
+  b->row_q = ok ? bq : q0;
+  b->C = ok ? bc : c0;
+  b->gx = b->nv / b->C;
+  if (!ok) return rc ? rc : fail(VPFP_EHIP, "autotune: no launch shape ran");
+  b->tuned_us = best;
+  return VPFP_OK;
+}
+
+int check_tuning(const vpfp_tuning* t, int nv) {
+  if (!t) return fail(VPFP_EBADARG, "null tuning");
+  if (t->row_q != 0 && t->row_q != 4 && t->row_q != 8) return fail(VPFP_EBADARG, "row_q must be 0, 4 or 8");
+  if (t->drift_tile != 0 && t->drift_tile != 4 && t->drift_tile != 8 && t->drift_tile != 16)
+    return fail(VPFP_EBADARG, "drift_tile must be 0, 4, 8 or 16");
+  if (t->drift_tile > nv) return fail(VPFP_EBADARG, "drift_tile exceeds nv");
+  if (t->fuse_field < -1 || t->fuse_field > 1) return fail(VPFP_EBADARG, "fuse_field must be -1, 0 or 1");
   return VPFP_OK;
 }
 
@@ -751,7 +932,8 @@ int vpfp_plan_create(const vpfp_grid_desc* d, vpfp_plan_t* out) {
   if (!rc) rc = upload(p->v, d->v, nvb);
   if (!rc) rc = upload(p->kv, d->kv, nvb);
   if (!rc) rc = p->prec == VPFP_F64 ? make_tables<double>(p) : make_tables<float>(p);
-  if (!rc) rc = dalloc(p, &p->dens, nxb * (size_t)(p->ntiles > 1 ? p->ntiles : 1));
+  // partial density rows for the narrowest drift tile any tuning may pick
+  if (!rc) rc = dalloc(p, &p->dens, nxb * (size_t)(p->C ? drift_grid(p->nv / 4) : 1));
   if (!rc) rc = dalloc(p, &p->ering, 2 * nxb);
   if (!rc) rc = dalloc(p, &p->series, 2 * sizeof(double));
   if (!rc) rc = dalloc(p, &p->rho_g, nxb);
@@ -855,12 +1037,12 @@ int vpfp_moments(vpfp_plan_t p, const void* f, double* moments, void* stream) {
     auto a = row_args<double>(p, f, nullptr);
     a.moments = moments;
     a.flags = vp::RK_MOMENTS;
-    return row_dispatch<double>(a, S(stream));
+    return row_dispatch<double>(a, S(stream), p->row_q);
   }
   auto a = row_args<float>(p, f, nullptr);
   a.moments = moments;
   a.flags = vp::RK_MOMENTS;
-  return row_dispatch<float>(a, S(stream));
+  return row_dispatch<float>(a, S(stream), p->row_q);
 }
 
 int vpfp_x_modes(vpfp_plan_t p, const void* f, int n_modes, double* out, void* stream) {
@@ -923,7 +1105,8 @@ int vpfp_batch_create(const vpfp_batch_desc* d, vpfp_batch_t* out) {
   if (!rc) rc = bput(b, &b->v, vv);
   if (!rc) rc = bput(b, &b->okx, okx);
   if (!rc) rc = b->prec == VPFP_F64 ? batch_tables<double>(b) : batch_tables<float>(b);
-  if (!rc) rc = balloc(b, &b->dens, sizeof(double) * (size_t)b->n * b->gx * b->nx);
+  // partial density rows for the narrowest drift tile any tuning may pick
+  if (!rc) rc = balloc(b, &b->dens, sizeof(double) * (size_t)b->n * (b->nv / 4) * b->nx);
   if (!rc) rc = balloc(b, &b->counters, sizeof(unsigned) * 16 * (size_t)b->n);
   if (!rc) rc = balloc(b, &b->ering, sizeof(double) * 2 * (size_t)b->n * b->nx);
   if (!rc) {
@@ -951,6 +1134,69 @@ int vpfp_batch_step_n(vpfp_batch_t b, void* f, double* e, int n_steps, double* s
   if (n_steps < 0) return fail(VPFP_EBADARG, "negative step count");
   return b->prec == VPFP_F64 ? batch_step_n<double>(b, f, e, n_steps, series, S(stream))
                              : batch_step_n<float>(b, f, e, n_steps, series, S(stream));
+}
+
+int vpfp_plan_get_tuning(vpfp_plan_t p, vpfp_tuning* t) {
+  if (int rc = check_plan(p)) return rc;
+  if (!t) return fail(VPFP_EBADARG, "null tuning");
+  t->row_q = p->row_q;
+  t->drift_tile = p->C;
+  t->fuse_field = p->fuse;
+  t->us_per_step = p->tuned_us;
+  return VPFP_OK;
+}
+
+int vpfp_plan_set_tuning(vpfp_plan_t p, const vpfp_tuning* t) {
+  if (int rc = check_plan(p)) return rc;
+  if (int rc = check_tuning(t, p->nv)) return rc;
+  p->row_q = t->row_q;
+  if (p->C && t->drift_tile) {
+    p->C = t->drift_tile;
+    p->ntiles = drift_grid(p->nv / p->C);
+  }
+  p->fuse = t->fuse_field;
+  return VPFP_OK;
+}
+
+int vpfp_plan_autotune(vpfp_plan_t p, int n_steps, void* stream, vpfp_tuning* chosen) {
+  if (int rc = check_plan(p)) return rc;
+  if (n_steps < 1) return fail(VPFP_EBADARG, "n_steps must be positive");
+  if (int rc = need_spectral_v(p)) return rc;
+  if (int rc = need_spectral_x(p)) return rc;
+  const int rc = p->prec == VPFP_F64 ? autotune_plan<double>(p, n_steps, S(stream))
+                                     : autotune_plan<float>(p, n_steps, S(stream));
+  if (rc) return rc;
+  return chosen ? vpfp_plan_get_tuning(p, chosen) : VPFP_OK;
+}
+
+int vpfp_batch_get_tuning(vpfp_batch_t b, vpfp_tuning* t) {
+  if (!b) return fail(VPFP_EBADARG, "null batch");
+  if (!t) return fail(VPFP_EBADARG, "null tuning");
+  t->row_q = b->row_q;
+  t->drift_tile = b->C;
+  t->fuse_field = 0;
+  t->us_per_step = b->tuned_us;
+  return VPFP_OK;
+}
+
+int vpfp_batch_set_tuning(vpfp_batch_t b, const vpfp_tuning* t) {
+  if (!b) return fail(VPFP_EBADARG, "null batch");
+  if (int rc = check_tuning(t, b->nv)) return rc;
+  b->row_q = t->row_q;
+  if (t->drift_tile) {
+    b->C = t->drift_tile;
+    b->gx = b->nv / b->C;
+  }
+  return VPFP_OK;
+}
+
+int vpfp_batch_autotune(vpfp_batch_t b, int n_steps, void* stream, vpfp_tuning* chosen) {
+  if (!b) return fail(VPFP_EBADARG, "null batch");
+  if (n_steps < 1) return fail(VPFP_EBADARG, "n_steps must be positive");
+  const int rc = b->prec == VPFP_F64 ? autotune_batch<double>(b, n_steps, S(stream))
+                                     : autotune_batch<float>(b, n_steps, S(stream));
+  if (rc) return rc;
+  return chosen ? vpfp_batch_get_tuning(b, chosen) : VPFP_OK;
 }
 
 int vpfp_plan_set_profiling(vpfp_plan_t p, int on) {

@@ -257,7 +257,27 @@ struct FieldArgs {
   double* e_out;          // [nx]
   double* e_out2;         // [nx] or null
   double* series;         // [2] or null: mean(E^2), mean(driver^2)
+  int series_stride;      // batched scan: doubles between consecutive instances' series entries
 };
+
+// The view of instance inst of a batched field solve: every array is
+// instance-major (part [inst][ntiles][nx], okx / driver / E [inst][nx],
+// series [inst * series_stride], one 64-byte counter slot per instance).
+// Instance 0 is the single-grid case itself.
+VP_HD FieldArgs field_instance(const FieldArgs& a, int inst) {
+  FieldArgs r = a;
+  if (inst == 0) return r;
+  const size_t xo = (size_t)inst * a.nx;
+  r.part += (size_t)inst * a.ntiles * a.nx;
+  if (r.rho_g) r.rho_g += xo;
+  if (r.counter) r.counter += inst * 16;
+  r.okx += xo;
+  if (r.driver) r.driver += xo;
+  r.e_out += xo;
+  if (r.e_out2) r.e_out2 += xo;
+  if (r.series) r.series += (size_t)inst * a.series_stride;
+  return r;
+}
 
 // rho[x] = 1 - scale * sum_tiles part[tile][x] for x in [x0, x0 + XS)
 // (all FT threads; XS divides FT or FT divides XS)
@@ -444,10 +464,10 @@ struct DriftArgs {
   double* dens;      // [gridDim][nx] partial sums (unscaled), or null
   int fuse_field;    // 1: the last workgroup to finish solves the field (fld)
   FieldArgs fld;     // field arguments for the fused tail (part = dens)
-  // batched scan: grid y = instance; f, dens, counter, okx, E and <E^2> are
-  // offset per instance; dk_inst[inst] replaces dk
+  // batched scan: grid y = instance; f and dens are offset per instance
+  // (the fused field tail takes field_instance of fld); dk_inst[inst]
+  // replaces dk
   const double* dk_inst;
-  int series_stride;  // doubles between consecutive instances' series entries
 };
 
 template <class T, int M, int Q, int C>
@@ -528,14 +548,7 @@ VP_HD void drift_body(Ctx& c, unsigned char* lds, const DriftArgs<T>& a) {
     if (a.fuse_field) {
       // last workgroup to arrive reduces every partial row and solves Poisson
       int* flag = reinterpret_cast<int*>(lds + Cfg::lds - 16);
-      FieldArgs fa = a.fld;
-      fa.part += (size_t)inst * c.nblk * a.nx;
-      fa.counter += inst * 16;
-      if (a.dk_inst) fa.okx += (size_t)inst * a.nx;
-      fa.e_out += (size_t)inst * a.nx;
-      if (fa.e_out2) fa.e_out2 += (size_t)inst * a.nx;
-      if (fa.driver) fa.driver += (size_t)inst * a.nx;
-      if (fa.series) fa.series += (size_t)inst * a.series_stride;
+      const FieldArgs fa = field_instance(a.fld, inst);
       c.fence();
       c.sync();
       if (c.tid == 0) *flag = (c.arrive(fa.counter) == (unsigned)(c.nblk - 1)) ? 1 : 0;

@@ -75,6 +75,12 @@ class _Class:
         self.f = (dens[:, :, None] * maxw[None, None, :]).to(_dev.f_dtype(precision)).contiguous()
         self.e = torch.zeros(self.n, self.nx, dtype=torch.float64, device=dev)
 
+    def autotune(self, n_steps, stream):
+        t = _lib.Tuning()
+        _lib.check(_lib.library().vpfp_batch_autotune(self.handle, int(n_steps), ctypes.c_void_p(stream.cuda_stream),
+                                                      ctypes.byref(t)))
+        return t.as_dict()
+
     def step(self, n_steps, series, stream):
         _lib.check(_lib.library().vpfp_batch_step_n(
             self.handle, ctypes.c_void_p(self.f.data_ptr()), ctypes.c_void_p(self.e.data_ptr()), int(n_steps),
@@ -108,6 +114,14 @@ class Scan:
             self.classes.append(_Class(idx, self.params, vmax, dt, op, precision, dev))
         self.streams = [torch.cuda.Stream() for _ in self.classes]
         self.cells = int(np.sum(self.params["nx"].astype(np.int64) * self.params["nv"].astype(np.int64)))
+
+    def autotune(self, n_steps=5):
+        """Pick each size class's launch shapes by timing them (scratch state;
+        the instances are not advanced).  Returns {(nx, nv): tuning}."""
+        out = {}
+        for c, s in zip(self.classes, self.streams):
+            out[(c.nx, c.nv)] = c.autotune(n_steps, s)
+        return out
 
     def run(self, n_steps, record=True):
         """Advance every instance n_steps.  Returns <E^2>(t) as [n_steps, n_inst]
