@@ -155,10 +155,12 @@ int vpfp_batch_destroy(vpfp_batch_t batch);
  * choices change rounding only, never the algorithm.
  *   row_q       values per thread of the v-direction row kernels: 4 or 8
  *               (0 = by row count: 4 when the grid has few rows)
- *   drift_tile  x-advection columns per workgroup: 4, 8 or 16 (0 = keep)
+ *   drift_tile  x-advection columns per workgroup: 4, 8, 16 or 32 (0 = keep)
  *   fuse_field  1: the last drift workgroup solves the field (nx <= 256);
  *               0 or -1 (default): separate field launch.  Batches always
  *               use the separate launch and report 0.
+ *   drift_order 1: consecutive column tiles on the workgroups of one XCD
+ *               (workgroup b runs on XCD b % 8); 0: tile = workgroup index
  *   us_per_step measured time of the chosen shape (autotune; else 0)
  * vpfp_plan_autotune / vpfp_batch_autotune time n_steps steps of every valid
  * shape on scratch state the call allocates and frees (the caller's f and E
@@ -167,6 +169,7 @@ typedef struct vpfp_tuning {
   int row_q;
   int drift_tile;
   int fuse_field;
+  int drift_order;
   double us_per_step;
 } vpfp_tuning;
 

@@ -1,10 +1,10 @@
 """Launch shapes and the autotuner.
 
 Every launch shape the autotuner may pick (row values per thread 4 or 8,
-drift tile of 4, 8 or 16 columns, field solve fused into the drift or on its
-own) must give the oracle's step to the FP64 bar (1e-12 in f, 1e-11 in E), and
-the autotuner must leave the plan on one of them without touching the
-caller's state.
+drift tile of 4, 8, 16 or 32 columns in workgroup or XCD order, field solve
+fused into the drift or on its own) must give the oracle's step to the FP64
+bar (1e-12 in f, 1e-11 in E), and the autotuner must leave the plan on one of
+them without touching the caller's state.
 """
 
 import itertools
@@ -50,22 +50,27 @@ def test_every_launch_shape_matches_oracle(nx, nv):
     g, f0, e0, p, dev = _setup(nx, nv)
     e_ref, f_ref = _oracle(g, f0, e0, n)
     ran = 0
-    for q, tile, fuse in itertools.product((4, 8), (4, 8, 16), (0, 1)):
-        print("shape", nx, nv, q, tile, fuse, flush=True)
-        p.set_tuning(row_q=q, drift_tile=tile, fuse_field=fuse)
+    for q, tile, fuse, order in itertools.product((4, 8), (4, 8, 16, 32), (0, 1), (0, 1)):
+        if tile > nv:
+            continue
+        print("shape", nx, nv, q, tile, fuse, order, flush=True)
+        p.set_tuning(row_q=q, drift_tile=tile, fuse_field=fuse, drift_order=order)
         f = torch.as_tensor(f0, device=dev).clone()
         e = torch.as_tensor(e0, device=dev).clone()
         try:
             ops.step_n(p, f, e, n)
         except NotImplementedError:
-            # this shape does not fit the grid (workgroup size, LDS, no tail)
-            assert fuse == 1 and nx > 256 or tile * (nx // 2) // 8 > 1024 or tile > nv
+            # the shape does not fit: drift workgroup above 1024 threads, or
+            # the fused field tail asked for nx > 256
+            m = nx // 2
+            wg = (m // (4 if m <= 64 else 8)) * tile
+            assert (fuse == 1 and nx > 256) or wg > 1024, (q, tile, fuse, order)
             continue
         torch.cuda.synchronize()
-        assert rel(f.cpu().numpy(), f_ref) < 1e-12, (q, tile, fuse)
-        assert rel(e.cpu().numpy(), e_ref) < 1e-11, (q, tile, fuse)
+        assert rel(f.cpu().numpy(), f_ref) < 1e-12, (q, tile, fuse, order)
+        assert rel(e.cpu().numpy(), e_ref) < 1e-11, (q, tile, fuse, order)
         ran += 1
-    assert ran >= 6
+    assert ran >= 8
 
 
 def test_autotune_picks_a_valid_shape_and_leaves_state_alone():
@@ -78,7 +83,8 @@ def test_autotune_picks_a_valid_shape_and_leaves_state_alone():
     e = torch.as_tensor(e0, device=dev).clone()
     t = p.autotune(n_steps=10)
     print("tuning", t, flush=True)
-    assert t["row_q"] in (4, 8) and t["drift_tile"] in (4, 8, 16) and t["fuse_field"] in (0, 1)
+    assert t["row_q"] in (4, 8) and t["drift_tile"] in (4, 8, 16, 32) and t["fuse_field"] in (0, 1)
+    assert t["drift_order"] in (0, 1)
     assert t["us_per_step"] > 0.0
     assert p.tuning() == t
     assert torch.equal(f, torch.as_tensor(f0, dtype=torch.float32, device=dev))
@@ -99,7 +105,7 @@ def test_batch_autotune_keeps_parity():
     print("batch tuning", tun, flush=True)
     assert set(tun) == {(32, 256), (64, 512)}
     for t in tun.values():
-        assert t["row_q"] in (4, 8) and t["drift_tile"] in (4, 8, 16) and t["us_per_step"] > 0.0
+        assert t["row_q"] in (4, 8) and t["drift_tile"] in (4, 8, 16, 32) and t["us_per_step"] > 0.0
     n = 10
     e2 = sc.run(n)
     for i in range(3):

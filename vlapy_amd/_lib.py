@@ -62,12 +62,13 @@ class Tuning(ctypes.Structure):
         ("row_q", _c_i),
         ("drift_tile", _c_i),
         ("fuse_field", _c_i),
+        ("drift_order", _c_i),
         ("us_per_step", _c_d),
     ]
 
     def as_dict(self):
         return {"row_q": self.row_q, "drift_tile": self.drift_tile, "fuse_field": self.fuse_field,
-                "us_per_step": self.us_per_step}
+                "drift_order": self.drift_order, "us_per_step": self.us_per_step}
 
 
 # name -> (argtypes); every function returns int status unless listed in _OTHER
@@ -185,7 +186,7 @@ class Plan:
         check(library().vpfp_plan_get_tuning(self.handle, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, row_q=None, drift_tile=None, fuse_field=None):
+    def set_tuning(self, row_q=None, drift_tile=None, fuse_field=None, drift_order=None):
         t = Tuning()
         check(library().vpfp_plan_get_tuning(self.handle, ctypes.byref(t)))
         if row_q is not None:
@@ -194,6 +195,8 @@ class Plan:
             t.drift_tile = int(drift_tile)
         if fuse_field is not None:
             t.fuse_field = int(fuse_field)
+        if drift_order is not None:
+            t.drift_order = int(drift_order)
         check(library().vpfp_plan_set_tuning(self.handle, ctypes.byref(t)))
 
     def autotune(self, n_steps=20, stream=None):

@@ -192,6 +192,7 @@ int launch_drift(const vp::DriftArgs<T>& a, int C, hipStream_t s, int gy) {
     case 4: return launch_drift_c<T, M, 4>(a, s, gy);
     case 8: return launch_drift_c<T, M, 8>(a, s, gy);
     case 16: return launch_drift_c<T, M, 16>(a, s, gy);
+    case 32: return launch_drift_c<T, M, 32>(a, s, gy);
   }
   return fail(VPFP_EUNSUPPORTED, "drift tile width");
 }
@@ -308,6 +309,7 @@ struct vpfp_plan_s {
   int C = 0, ntiles = 0;
   int row_q = 0;   // values per thread of the row kernels (0: row-count rule)
   int fuse = -1;   // field solved by the last drift workgroup: 1 yes, 0 no, -1 size rule
+  int xcd = 1;     // drift tile order (xcd_block); never slower in the measured shapes
   double tuned_us = 0.0;
   double *kx = nullptr, *okx = nullptr, *v = nullptr, *kv = nullptr;
   void* tw_v = nullptr;          // cx<T> [lo | hi] tables for nv
@@ -489,6 +491,7 @@ vp::DriftArgs<T> drift_args(vpfp_plan_t p, const void* fin, void* fout, double d
   a.fuse_field = 0;
   std::memset(&a.fld, 0, sizeof(a.fld));
   a.dk_inst = nullptr;
+  a.xcd_order = p->xcd;
   return a;
 }
 
@@ -627,6 +630,7 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
 struct vpfp_batch_s {
   int n = 0, nx = 0, nv = 0, prec = VPFP_F64, op = VPFP_OP_DG, C = 0, gx = 0;
   int row_q = 0;
+  int xcd = 1;
   double tuned_us = 0.0;
   double dv = 0, dt = 0, v0 = 0, dvv = 0, dk_v = 0;
   double *dk_x = nullptr, *okx = nullptr, *nu_dt = nullptr, *v = nullptr;
@@ -706,6 +710,7 @@ int batch_step_n(vpfp_batch_s* b, void* f, double* e, int n, double* series, hip
     da.tw = reinterpret_cast<const cx<T>*>(b->tw_x);
     da.dens = b->dens;
     da.dk_inst = b->dk_x;
+    da.xcd_order = b->xcd;
     // the field is a launch of its own: a last-workgroup hand-off inside the
     // drift would need a device-scope fence (an L2 write-back and invalidate)
     // in each of the n * gx drift workgroups
@@ -798,29 +803,33 @@ template <class T>
 int autotune_plan(vpfp_plan_t p, int n_steps, hipStream_t s) {
   TuneScratch w;
   if (int rc = w.init(tune_state<T>(p->nx, p->nx, p->nv, p->v0, p->dvv), 2 * (size_t)p->nx)) return rc;
-  const int q0 = p->row_q, c0 = p->C, f0 = p->fuse;
+  const int q0 = p->row_q, c0 = p->C, f0 = p->fuse, x0 = p->xcd;
   const bool prof = p->profiling;
   p->profiling = false;
   double best = 1e300;
-  int bq = q0, bc = c0, bf = f0, rc = VPFP_OK;
+  int bq = q0, bc = c0, bf = f0, bx = x0, rc = VPFP_OK;
   for (int q : row_q_candidates(p->nv)) {
-    for (int C : {4, 8, 16}) {
+    for (int C : {4, 8, 16, 32}) {
       if (C > p->nv) continue;
       for (int fuse : {0, 1}) {
         if (fuse && p->nx > 256) continue;
-        p->row_q = q;
-        p->C = C;
-        p->ntiles = drift_grid(p->nv / C);
-        p->fuse = fuse;
-        double us = 0.0;
-        rc = w.time([&](int n) { return do_step_n<T>(p, w.f, w.e, n, nullptr, nullptr, nullptr, nullptr, s); },
-                    n_steps, s, &us);
-        if (rc == VPFP_EUNSUPPORTED) continue;  // shape does not fit this grid
-        if (rc) break;
-        if (us < best) {
-          best = us;
-          bq = q, bc = C, bf = fuse;
+        for (int xo : {0, 1}) {
+          p->row_q = q;
+          p->C = C;
+          p->ntiles = drift_grid(p->nv / C);
+          p->fuse = fuse;
+          p->xcd = xo;
+          double us = 0.0;
+          rc = w.time([&](int n) { return do_step_n<T>(p, w.f, w.e, n, nullptr, nullptr, nullptr, nullptr, s); },
+                      n_steps, s, &us);
+          if (rc == VPFP_EUNSUPPORTED) continue;  // shape does not fit this grid
+          if (rc) break;
+          if (us < best) {
+            best = us;
+            bq = q, bc = C, bf = fuse, bx = xo;
+          }
         }
+        if (rc && rc != VPFP_EUNSUPPORTED) break;
       }
       if (rc && rc != VPFP_EUNSUPPORTED) break;
     }
@@ -832,6 +841,7 @@ int autotune_plan(vpfp_plan_t p, int n_steps, hipStream_t s) {
   p->C = ok ? bc : c0;
   p->ntiles = drift_grid(p->nv / p->C);
   p->fuse = ok ? bf : f0;
+  p->xcd = ok ? bx : x0;
   if (!ok) return rc ? rc : fail(VPFP_EHIP, "autotune: no launch shape ran");
   p->tuned_us = best;
   return VPFP_OK;
@@ -842,23 +852,27 @@ int autotune_batch(vpfp_batch_s* b, int n_steps, hipStream_t s) {
   TuneScratch w;
   const size_t rows = (size_t)b->n * b->nx;
   if (int rc = w.init(tune_state<T>(rows, b->nx, b->nv, b->v0, b->dvv), rows)) return rc;
-  const int q0 = b->row_q, c0 = b->C;
+  const int q0 = b->row_q, c0 = b->C, x0 = b->xcd;
   double best = 1e300;
-  int bq = q0, bc = c0, rc = VPFP_OK;
+  int bq = q0, bc = c0, bx = x0, rc = VPFP_OK;
   for (int q : row_q_candidates(b->nv)) {
-    for (int C : {4, 8, 16}) {
+    for (int C : {4, 8, 16, 32}) {
       if (C > b->nv) continue;
-      b->row_q = q;
-      b->C = C;
-      b->gx = b->nv / C;
-      double us = 0.0;
-      rc = w.time([&](int n) { return batch_step_n<T>(b, w.f, w.e, n, nullptr, s); }, n_steps, s, &us);
-      if (rc == VPFP_EUNSUPPORTED) continue;
-      if (rc) break;
-      if (us < best) {
-        best = us;
-        bq = q, bc = C;
+      for (int xo : {0, 1}) {
+        b->row_q = q;
+        b->C = C;
+        b->gx = b->nv / C;
+        b->xcd = xo;
+        double us = 0.0;
+        rc = w.time([&](int n) { return batch_step_n<T>(b, w.f, w.e, n, nullptr, s); }, n_steps, s, &us);
+        if (rc == VPFP_EUNSUPPORTED) continue;
+        if (rc) break;
+        if (us < best) {
+          best = us;
+          bq = q, bc = C, bx = xo;
+        }
       }
+      if (rc && rc != VPFP_EUNSUPPORTED) break;
     }
     if (rc && rc != VPFP_EUNSUPPORTED) break;
   }
@@ -866,6 +880,7 @@ int autotune_batch(vpfp_batch_s* b, int n_steps, hipStream_t s) {
   b->row_q = ok ? bq : q0;
   b->C = ok ? bc : c0;
   b->gx = b->nv / b->C;
+  b->xcd = ok ? bx : x0;
   if (!ok) return rc ? rc : fail(VPFP_EHIP, "autotune: no launch shape ran");
   b->tuned_us = best;
   return VPFP_OK;
@@ -874,8 +889,9 @@ int autotune_batch(vpfp_batch_s* b, int n_steps, hipStream_t s) {
 int check_tuning(const vpfp_tuning* t, int nv) {
   if (!t) return fail(VPFP_EBADARG, "null tuning");
   if (t->row_q != 0 && t->row_q != 4 && t->row_q != 8) return fail(VPFP_EBADARG, "row_q must be 0, 4 or 8");
-  if (t->drift_tile != 0 && t->drift_tile != 4 && t->drift_tile != 8 && t->drift_tile != 16)
-    return fail(VPFP_EBADARG, "drift_tile must be 0, 4, 8 or 16");
+  if (t->drift_tile != 0 && t->drift_tile != 4 && t->drift_tile != 8 && t->drift_tile != 16 && t->drift_tile != 32)
+    return fail(VPFP_EBADARG, "drift_tile must be 0, 4, 8, 16 or 32");
+  if (t->drift_order != 0 && t->drift_order != 1) return fail(VPFP_EBADARG, "drift_order must be 0 or 1");
   if (t->drift_tile > nv) return fail(VPFP_EBADARG, "drift_tile exceeds nv");
   if (t->fuse_field < -1 || t->fuse_field > 1) return fail(VPFP_EBADARG, "fuse_field must be -1, 0 or 1");
   return VPFP_OK;
@@ -1142,6 +1158,7 @@ int vpfp_plan_get_tuning(vpfp_plan_t p, vpfp_tuning* t) {
   t->row_q = p->row_q;
   t->drift_tile = p->C;
   t->fuse_field = p->fuse;
+  t->drift_order = p->xcd;
   t->us_per_step = p->tuned_us;
   return VPFP_OK;
 }
@@ -1155,6 +1172,7 @@ int vpfp_plan_set_tuning(vpfp_plan_t p, const vpfp_tuning* t) {
     p->ntiles = drift_grid(p->nv / p->C);
   }
   p->fuse = t->fuse_field;
+  p->xcd = t->drift_order;
   return VPFP_OK;
 }
 
@@ -1175,6 +1193,7 @@ int vpfp_batch_get_tuning(vpfp_batch_t b, vpfp_tuning* t) {
   t->row_q = b->row_q;
   t->drift_tile = b->C;
   t->fuse_field = 0;
+  t->drift_order = b->xcd;
   t->us_per_step = b->tuned_us;
   return VPFP_OK;
 }
@@ -1187,6 +1206,7 @@ int vpfp_batch_set_tuning(vpfp_batch_t b, const vpfp_tuning* t) {
     b->C = t->drift_tile;
     b->gx = b->nv / b->C;
   }
+  b->xcd = t->drift_order;
   return VPFP_OK;
 }
 

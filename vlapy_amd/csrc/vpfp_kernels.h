@@ -468,7 +468,13 @@ struct DriftArgs {
   // (the fused field tail takes field_instance of fld); dk_inst[inst]
   // replaces dk
   const double* dk_inst;
+  int xcd_order;  // 1: consecutive column tiles go to workgroups on one XCD
 };
+
+// Workgroup b of n runs on XCD b % 8 (round-robin dispatch).  With
+// xcd_order the workgroups of one XCD take consecutive tiles, so the two
+// halves of a 128-byte line read by neighbouring tiles meet in one L2.
+VP_HD int xcd_block(int b, int n) { return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3); }
 
 template <class T, int M, int Q, int C>
 struct DriftCfg {
@@ -504,7 +510,8 @@ VP_HD void drift_body(Ctx& c, unsigned char* lds, const DriftArgs<T>& a) {
   const int ntiles = a.nv / C;
   // the grid strides over the column tiles; workgroup b accumulates the
   // density of every tile it advects into one partial row dens[b][x]
-  for (int tile = c.bid; tile < ntiles; tile += c.nblk) {
+  const int b0 = a.xcd_order ? xcd_block(c.bid, c.nblk) : c.bid;
+  for (int tile = b0; tile < ntiles; tile += c.nblk) {
     const int col = tile * C + cc;
     const T* fi = a.f_in + foff + col;
     cx<T> v[Q];
