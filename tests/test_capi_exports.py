@@ -8,21 +8,25 @@ import re
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "vpfp.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("vpfp.h", "vpfc.h")]
 LIB = os.path.join(ROOT, "vlapy_amd", "libvpfp.so")
 
 
 def declared():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(vpfp_[a-z_0-9]+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"\b(vpf[pc]_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_declares_the_boundary():
     names = declared()
     for must in ("vpfp_plan_create", "vpfp_plan_destroy", "vpfp_vdfdx", "vpfp_edfdv", "vpfp_field",
                  "vpfp_solve_for_field", "vpfp_compute_charges", "vpfp_collide", "vpfp_collision_diagonals",
-                 "vpfp_tridiag_solve", "vpfp_step_n", "vpfp_moments", "vpfp_x_modes", "vpfp_last_error"):
+                 "vpfp_tridiag_solve", "vpfp_step_n", "vpfp_moments", "vpfp_x_modes", "vpfp_last_error",
+                 "vpfp_plan_autotune", "vpfp_batch_autotune", "vpfc_create", "vpfc_apply", "vpfc_train_step",
+                 "vpfc_gradients", "vpfp_batch_set_closure"):
         assert must in names
 
 

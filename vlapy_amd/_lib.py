@@ -99,8 +99,21 @@ SIGNATURES = {
     "vpfp_batch_autotune": [_c_p, _c_i, _c_p, ctypes.POINTER(Tuning)],
     "vpfp_last_error": [],
     "vpfp_version": [],
+    # learned closure (include/vpfc.h)
+    "vpfc_create": [_c_i, ctypes.POINTER(_c_p)],
+    "vpfc_destroy": [_c_p],
+    "vpfc_set_params": [_c_p, _c_p],
+    "vpfc_get_params": [_c_p, _c_p],
+    "vpfc_set_scales": [_c_p, ctypes.c_float, ctypes.c_float],
+    "vpfc_forward": [_c_p, _c_p, _c_i, _c_p, _c_p],
+    "vpfc_apply": [_c_p, _c_p, _c_i, _c_p],
+    "vpfc_gradients": [_c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p],
+    "vpfc_train_step": [_c_p, _c_p, _c_p, _c_i, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                        _c_p, _c_p],
+    "vpfp_batch_set_closure": [_c_p, _c_p],
+    "vpfc_last_error": [],
 }
-_OTHER = {"vpfp_last_error": ctypes.c_char_p, "vpfp_version": _c_i}
+_OTHER = {"vpfp_last_error": ctypes.c_char_p, "vpfp_version": _c_i, "vpfc_last_error": ctypes.c_char_p}
 
 _lib = None
 
@@ -122,11 +135,13 @@ def library():
     return _lib
 
 
-def check(status):
-    """Map a C ABI status to the reference's Python exceptions."""
+def check(status, closure=False):
+    """Map a C ABI status to the reference's Python exceptions (closure:
+    the message is vpfc_last_error's)."""
     if status == VPFP_OK:
         return
-    msg = library().vpfp_last_error().decode(errors="replace")
+    err = library().vpfc_last_error if closure else library().vpfp_last_error
+    msg = err().decode(errors="replace")
     if status == VPFP_EBADARG:
         raise ValueError(msg)
     if status == VPFP_EUNSUPPORTED:

@@ -9,7 +9,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "vpfp.hip")
+SRCS = [os.path.join(HERE, "csrc", "vpfp.hip"), os.path.join(HERE, "csrc", "vpfc.hip")]
 OUT = os.path.join(HERE, "libvpfp.so")
 
 HIPCC_FLAGS = [
@@ -31,9 +31,9 @@ def hipcc():
 
 
 def build(verbose=True):
-    """Compile vlapy_amd/csrc/vpfp.hip into vlapy_amd/libvpfp.so (always recompiles)."""
+    """Compile vlapy_amd/csrc/*.hip into vlapy_amd/libvpfp.so (always recompiles)."""
     tmp = OUT + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-I" + os.path.join(ROOT, "include"), SRC, "-o", tmp]
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-I" + os.path.join(ROOT, "include")] + SRCS + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "vpfp.h"
+#include "vpfc.h"
 #include "vpfp_kernels.h"
 
 using vp::cx;
@@ -632,6 +633,7 @@ struct vpfp_batch_s {
   int row_q = 0;
   int xcd = 1;
   double tuned_us = 0.0;
+  vpfc_net_t closure = nullptr;  // learned closure after every step (FP32, nv 256)
   double dv = 0, dt = 0, v0 = 0, dvv = 0, dk_v = 0;
   double *dk_x = nullptr, *okx = nullptr, *nu_dt = nullptr, *v = nullptr;
   void *tw_v = nullptr, *tw_x = nullptr;
@@ -697,7 +699,14 @@ int batch_step_n(vpfp_batch_s* b, void* f, double* e, int n, double* series, hip
   if (n > 0)
     if (int rc = row_dispatch<T>(ra, s, b->row_q)) return rc;
   const double* e_cur = e;
+  const bool closure = b->closure != nullptr;
   for (int i = 0; i < n; ++i) {
+    if (closure && i > 0) {
+      // first half kick of this step (not fused: the closure ran in between)
+      ra.e = e_cur;
+      ra.flags = vp::RK_KICK_A;
+      if (int rc = row_dispatch<T>(ra, s, b->row_q)) return rc;
+    }
     double* ei = b->ering + (size_t)(i & 1) * b->n * b->nx;
     vp::DriftArgs<T> da;
     std::memset(&da, 0, sizeof(da));
@@ -730,8 +739,13 @@ int batch_step_n(vpfp_batch_s* b, void* f, double* e, int n, double* series, hip
     if (int rc = field_dispatch(fa, true, s, b->n)) return rc;
     e_cur = ei;
     ra.e = e_cur;
-    ra.flags = vp::RK_KICK_A | vp::RK_COLLIDE | (i + 1 < n ? vp::RK_KICK_B : 0);
+    ra.flags = vp::RK_KICK_A | vp::RK_COLLIDE | (i + 1 < n && !closure ? vp::RK_KICK_B : 0);
     if (int rc = row_dispatch<T>(ra, s, b->row_q)) return rc;
+    if (closure) {
+      if (int rc = vpfc_apply(b->closure, reinterpret_cast<float*>(f), b->n * b->nx, s)) {
+        return fail(rc, std::string("closure: ") + vpfc_last_error());
+      }
+    }
   }
   if (n > 0) VPFP_HIP(hipMemcpyAsync(e, e_cur, sizeof(double) * b->n * b->nx, hipMemcpyDeviceToDevice, s));
   return VPFP_OK;
@@ -1134,6 +1148,14 @@ int vpfp_batch_create(const vpfp_batch_desc* d, vpfp_batch_t* out) {
     return rc;
   }
   *out = b;
+  return VPFP_OK;
+}
+
+int vpfp_batch_set_closure(vpfp_batch_t b, vpfc_net_t net) {
+  if (!b) return fail(VPFP_EBADARG, "null batch");
+  if (net && (b->prec != VPFP_F32 || b->nv != VPFC_NV))
+    return fail(VPFP_EUNSUPPORTED, "the closure runs on FP32 batches with nv = " + std::to_string(VPFC_NV));
+  b->closure = net;
   return VPFP_OK;
 }
 

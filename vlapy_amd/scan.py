@@ -89,7 +89,7 @@ class _Class:
 
     def __del__(self):
         h = getattr(self, "handle", None)
-        if h is not None and h.value and _lib._lib is not None:
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
             _lib._lib.vpfp_batch_destroy(h)
 
 
