@@ -102,7 +102,7 @@ def cpu_baseline(target_s):
 # --------------------------------------------------------------------------- GPU
 
 
-def gpu_run(precision, steps, warmup, profile_steps):
+def gpu_run(precision, steps, warmup, profile_steps, autotune=True):
     import numpy as np
     import torch
 
@@ -113,6 +113,8 @@ def gpu_run(precision, steps, warmup, profile_steps):
     x, kx, okx, v, kv, dv, f0 = initial_state(np)
     plan = Plan(NX, NV, precision, kx, okx, v, kv, dv, NU, DT, "dg")
     fdt = ops.f_dtype(plan)
+    # launch shapes picked by timing them on scratch state (untimed setup)
+    tuning = plan.autotune(50) if autotune else plan.tuning()
 
     def fresh():
         return (torch.as_tensor(f0, dtype=fdt, device=dev).contiguous(),
@@ -156,31 +158,39 @@ def gpu_run(precision, steps, warmup, profile_steps):
     result["kernels"] = plan.kernel_times()
     plan.set_profiling(False)
     result["bytes_f"] = 8 if precision == "fp64" else 4
-    result["ntiles"] = NV // 16
+    result["ntiles"] = NV // tuning["drift_tile"]
+    result["tuning"] = tuning
     return result
 
 
-def roofline(run):
-    """Dominant kernel: achieved algorithmic GB/s vs the 8 TB/s HBM peak."""
-    cells = NX * NV
+def roofline(run, cells, ms_per_step, traffic_file=None):
+    """Dominant kernel: achieved algorithmic GB/s vs the 8 TB/s HBM peak.
+
+    Algorithmic bytes are one read and one write of f per split operator
+    (moments, Poisson and the storage epilogue fused: no extra bytes), s = 4
+    (FP32 f) or 8 (FP64 f) bytes per cell:
+      row kernel   kick(dt/2) + collision + kick(dt/2) = 3 operators: 6 s cells
+      drift        one operator: 2 s cells
+      field        the partial densities it reads and E: 8 (ntiles + 1) nx
+    and the whole step (leapfrog + collision, 4 operators) 8 s per cell-update.
+    hbm_min_bytes_per_launch is what the fused kernel itself must move."""
     s = run["bytes_f"]
     kern = run["kernels"]
     dom = max(("row", "drift", "field"), key=lambda k: kern[k][0])
     total_ms, launches = kern[dom]
     avg_s = total_ms / launches / 1e3
-    # algorithmic bytes per launch: one read and one write of f (plus the
-    # partial densities for the drift; E and moments for the row kernel)
-    alg = {
-        "row": 2 * s * cells + 8 * NX * 9,
-        "drift": 2 * s * cells + 8 * NX * run["ntiles"],
-        "field": 8 * NX * run["ntiles"] + 8 * NX,
-    }[dom]
+    nx = run.get("nx", NX)
+    alg = {"row": 6 * s * cells, "drift": 2 * s * cells, "field": 8 * (run["ntiles"] + 1) * nx}[dom]
+    hbm_min = {"row": 2 * s * cells, "drift": 2 * s * cells + 8 * nx * run["ntiles"],
+               "field": 8 * (run["ntiles"] + 1) * nx}[dom]
     achieved = alg / avg_s / 1e9
     traffic = None
-    prof = os.path.join(HERE, "profiles", "traffic_c2_%s.json" % ("fp64" if s == 8 else "fp32"))
-    if os.path.exists(prof):
-        with open(prof) as fh:
-            traffic = json.load(fh).get(dom)
+    if traffic_file:
+        prof = os.path.join(HERE, "profiles", traffic_file)
+        if os.path.exists(prof):
+            with open(prof) as fh:
+                traffic = json.load(fh).get(dom)
+    step_gbs = 8 * s * cells / (ms_per_step / 1e3) / 1e9
     return {
         "bound": "hbm",
         "kernel": dom,
@@ -190,8 +200,12 @@ def roofline(run):
         "frac": achieved / HBM_PEAK_GBS,
         "traffic": traffic,
         "alg_bytes_per_launch": alg,
+        "hbm_min_bytes_per_launch": hbm_min,
+        "hbm_min_frac": hbm_min / avg_s / 1e9 / HBM_PEAK_GBS,
         "avg_launch_us": avg_s * 1e6,
         "kernel_us": {k: (v[0] / v[1] * 1e3 if v[1] else None) for k, v in kern.items()},
+        "step_alg_GBps": step_gbs,
+        "step_frac": step_gbs / HBM_PEAK_GBS,
     }
 
 
@@ -226,12 +240,13 @@ def scan_run(precision, steps):
     torch.cuda.synchronize()
     del sc
     sc = Scan(p, dt=SCAN_DT, op="dg", precision=precision)
+    tuning = sc.autotune(3)  # launch shapes per size class (untimed setup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e2 = sc.run(steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return p, sc.cells, e2, wall
+    return p, sc.cells, e2, wall, tuning
 
 
 def scan_cpu(p, steps_sample=10, n_sample=12):
@@ -281,8 +296,8 @@ def scan_section(steps, with_cpu):
 
     from vlapy_amd.diagnostics.low_level_helpers import get_standing_wave_rate_and_frequency as rate
 
-    p, cells, e2_32, wall32 = scan_run("fp32", steps)
-    _, _, e2_64, wall64 = scan_run("fp64", steps)
+    p, cells, e2_32, wall32, tun32 = scan_run("fp32", steps)
+    _, _, e2_64, wall64, _ = scan_run("fp64", steps)
     t = SCAN_DT * (np.arange(steps) + 1)
     errs = []
     for i in range(SCAN_N):
@@ -302,6 +317,12 @@ def scan_section(steps, with_cpu):
         "fp64_instances_per_s": SCAN_N / wall64,
         "fp64_seconds": wall64,
         "cells": cells,
+        # algorithmic bytes of a leapfrog + collision step: 8 s per cell-update
+        "roofline": {"bound": "hbm", "achieved": 32.0 * cells * steps / wall32 / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": 32.0 * cells * steps / wall32 / 1e9 / HBM_PEAK_GBS,
+                     "fp64_achieved": 64.0 * cells * steps / wall64 / 1e9,
+                     "fp64_frac": 64.0 * cells * steps / wall64 / 1e9 / HBM_PEAK_GBS},
+        "tuning": {"%dx%d" % k: v for k, v in tun32.items()},
         "damping_rate": {
             "instances_fitted": int(errs.size),
             "within_1pct": float(np.mean(errs <= 0.01)) if errs.size else None,
@@ -323,6 +344,75 @@ def scan_section(steps, with_cpu):
     return out
 
 
+# --------------------------------------------------------------------------- large grid
+
+LG_NX, LG_NV, LG_K, LG_VMAX, LG_AMP, LG_NU, LG_DT = 1024, 8192, 0.35, 8.0, 0.1, 1e-4, 0.02
+
+
+def large_state(np):
+    """nx=1024 on [0, 2 pi/k), k=0.35; nv=8192 on [-8, 8]; f0 = (1 + 0.1 sum_{m=1..3}
+    cos(m k x + phi_m)) exp(-v^2/2)/sqrt(2 pi), phi_m ~ U[0, 2 pi) from default_rng(4)."""
+    xmax = 2 * np.pi / LG_K
+    dx = xmax / LG_NX
+    x = np.linspace(dx / 2.0, xmax - dx / 2.0, LG_NX)
+    kx = np.fft.fftfreq(LG_NX, d=dx) * 2.0 * np.pi
+    okx = np.zeros_like(kx)
+    okx[1:] = 1.0 / kx[1:]
+    dv = 2 * LG_VMAX / LG_NV
+    v = np.linspace(-LG_VMAX + dv / 2.0, LG_VMAX - dv / 2.0, LG_NV)
+    kv = np.fft.fftfreq(LG_NV, d=dv) * 2.0 * np.pi
+    phi = np.random.default_rng(4).uniform(0.0, 2 * np.pi, 3)
+    n = 1.0 + LG_AMP * sum(np.cos(m * LG_K * x + phi[m - 1]) for m in (1, 2, 3))
+    f0 = n[:, None] * (np.exp(-v ** 2 / 2.0) / np.sqrt(2 * np.pi))[None, :]
+    return x, kx, okx, v, kv, dv, f0
+
+
+def large_section(steps, autotune=True):
+    """Single high-resolution run (8.4M cells), Dougherty nu=1e-4, dt=0.02:
+    `steps` leapfrog + collision steps with the E history, FP32 and FP64."""
+    import numpy as np
+    import torch
+
+    from vlapy_amd import ops
+    from vlapy_amd._lib import Plan
+
+    dev = torch.device("cuda:0")
+    x, kx, okx, v, kv, dv, f0 = large_state(np)
+    cells = LG_NX * LG_NV
+    out = {"workload": "nx=1024, nv=8192, v in [-8, 8], k=0.35, three harmonics of amplitude 0.1 with phases from "
+                       "default_rng(4), Dougherty nu=1e-4, dt=0.02; %d steps timed (leapfrog + collision + E "
+                       "history)" % steps}
+    for prec in ("fp32", "fp64"):
+        plan = Plan(LG_NX, LG_NV, prec, kx, okx, v, kv, dv, LG_NU, LG_DT, "dg")
+        tuning = plan.autotune(3) if autotune else plan.tuning()
+        fdt = ops.f_dtype(plan)
+        f = torch.as_tensor(f0, dtype=fdt, device=dev).contiguous()
+        e = torch.zeros(LG_NX, dtype=torch.float64, device=dev)
+        eh = torch.zeros(steps, LG_NX, dtype=torch.float64, device=dev)
+        ops.step_n(plan, f, e, 5, None, eh[:5])
+        f = torch.as_tensor(f0, dtype=fdt, device=dev).contiguous()
+        e.zero_()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        ops.step_n(plan, f, e, steps, None, eh)
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / steps
+        if not torch.isfinite(eh).all():
+            raise RuntimeError("non-finite field history (large grid)")
+        plan.set_profiling(True)
+        ops.step_n(plan, f, e, 20, None, eh[:20])
+        torch.cuda.synchronize()
+        run = {"kernels": plan.kernel_times(), "bytes_f": 8 if prec == "fp64" else 4,
+               "ntiles": min(LG_NV // tuning["drift_tile"], 1024), "nx": LG_NX}
+        plan.set_profiling(False)
+        out[prec] = {"cell_updates_per_s": cells / (ms / 1e3), "ms_per_step": ms, "tuning": tuning,
+                     "roofline": roofline(run, cells, ms)}
+        del plan, f
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -333,6 +423,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scan-steps", type=int, default=SCAN_STEPS)
     ap.add_argument("--no-scan", action="store_true")
+    ap.add_argument("--large-steps", type=int, default=200)
+    ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes (for counter runs that should see only the timed shapes)")
+    ap.add_argument("--copy-bytes", type=int, default=None,
+                    help="only copy a device buffer of this many bytes 5 times (HBM counter calibration)")
     ap.add_argument("--cpu-child", type=float, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--scan-cpu-child", type=str, default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -343,14 +439,25 @@ def main():
     if args.scan_cpu_child is not None:
         scan_cpu_child(args.scan_cpu_child)
         return 0
+    if args.copy_bytes is not None:
+        import torch
+
+        src = torch.ones(args.copy_bytes // 4, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 4, "copies": 5}))
+        return 0
     if args.gpus != 1:
-        print("bench.py measures one GPU only: the build has no multi-GPU path (instances of the "
-              "parameter scan shard with no exchange; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the build has no multi-GPU path (the parameter scan "
+              "would shard its instances over GPUs with no exchange)", file=sys.stderr)
         return 2
 
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
-    r32 = gpu_run("fp32", args.steps, args.warmup, min(args.profile_steps, args.steps))
-    r64 = gpu_run("fp64", args.steps, args.warmup, min(args.profile_steps, args.steps))
+    at = not args.no_autotune
+    r32 = gpu_run("fp32", args.steps, args.warmup, min(args.profile_steps, args.steps), at)
+    r64 = gpu_run("fp64", args.steps, args.warmup, min(args.profile_steps, args.steps), at)
 
     import numpy as np
 
@@ -378,17 +485,21 @@ def main():
             "grid": [NX, NV],
             "parallelism": "single GPU",
         },
-        "roofline": roofline(r32),
+        "roofline": roofline(r32, cells, r32["ms"] / args.steps, "traffic_c2_fp32.json"),
         "cpu_baseline": cpu,
+        "tuning": r32["tuning"],
         "fp64": {
             "value": cells * args.steps / (r64["ms"] / 1e3),
             "ms_per_step": r64["ms"] / args.steps,
-            "roofline": roofline(r64),
+            "roofline": roofline(r64, cells, r64["ms"] / args.steps, "traffic_c2_fp64.json"),
+            "tuning": r64["tuning"],
         },
         "f32_vs_f64_after_run": {"E_max_rel": err_e, "f_max_rel": err_f},
     }
     if not args.no_scan:
         out["scan"] = scan_section(args.scan_steps, not args.no_cpu)
+    if not args.no_large:
+        out["large_grid"] = large_section(args.large_steps, at)
     print(json.dumps(out))
     return 0
 

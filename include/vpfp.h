@@ -153,14 +153,18 @@ int vpfp_batch_destroy(vpfp_batch_t batch);
 /* Launch shapes of the step and the autotuner that picks them.  No
  * counterpart in the reference (its NumPy path has no launch shapes); the
  * choices change rounding only, never the algorithm.
- *   row_q       values per thread of the v-direction row kernels: 4 or 8
- *               (0 = by row count: 4 when the grid has few rows)
+ *   row_q       values per thread of the v-direction row kernels: 4
+ *               (16 <= nv/2 <= 2048), 8, or 16 (nv/2 >= 256); 0 = by row
+ *               count (4 when the grid has few rows, else 8)
  *   drift_tile  x-advection columns per workgroup: 4, 8, 16 or 32 (0 = keep)
  *   fuse_field  1: the last drift workgroup solves the field (nx <= 256);
  *               0 or -1 (default): separate field launch.  Batches always
  *               use the separate launch and report 0.
  *   drift_order 1: consecutive column tiles on the workgroups of one XCD
  *               (workgroup b runs on XCD b % 8); 0: tile = workgroup index
+ *   drift_tiles column tiles per drift workgroup (1 = one each; a single
+ *               grid never uses more than 1024 workgroups): fewer
+ *               workgroups write fewer partial density rows (0 = keep)
  *   us_per_step measured time of the chosen shape (autotune; else 0)
  * vpfp_plan_autotune / vpfp_batch_autotune time n_steps steps of every valid
  * shape on scratch state the call allocates and frees (the caller's f and E
@@ -170,6 +174,7 @@ typedef struct vpfp_tuning {
   int drift_tile;
   int fuse_field;
   int drift_order;
+  int drift_tiles;
   double us_per_step;
 } vpfp_tuning;
 

@@ -1,8 +1,8 @@
 """Launch shapes and the autotuner.
 
-Every launch shape the autotuner may pick (row values per thread 4 or 8,
-drift tile of 4, 8, 16 or 32 columns in workgroup or XCD order, field solve
-fused into the drift or on its own) must give the oracle's step to the FP64
+Every launch shape the autotuner may pick (row values per thread 4, 8 or 16,
+drift tile of 4, 8, 16 or 32 columns in workgroup or XCD order, 1, 2 or 4
+tiles per drift workgroup, field solve fused into the drift or on its own) must give the oracle's step to the FP64
 bar (1e-12 in f, 1e-11 in E), and the autotuner must leave the plan on one of
 them without touching the caller's state.
 """
@@ -41,7 +41,7 @@ def _oracle(g, f0, e0, n, dt=0.1, nu=1e-2, op="dg"):
     return e, f
 
 
-@pytest.mark.parametrize("nx,nv", [(16, 64), (256, 64), (64, 1024), (512, 32)])
+@pytest.mark.parametrize("nx,nv", [(16, 64), (256, 64), (64, 1024), (512, 32), (8, 2048)])
 def test_every_launch_shape_matches_oracle(nx, nv):
     import torch
     from vlapy_amd import ops
@@ -50,11 +50,11 @@ def test_every_launch_shape_matches_oracle(nx, nv):
     g, f0, e0, p, dev = _setup(nx, nv)
     e_ref, f_ref = _oracle(g, f0, e0, n)
     ran = 0
-    for q, tile, fuse, order in itertools.product((4, 8), (4, 8, 16, 32), (0, 1), (0, 1)):
+    for q, tile, fuse, order, per in itertools.product((4, 8, 16), (4, 8, 16, 32), (0, 1), (0, 1), (1, 2, 4)):
         if tile > nv:
             continue
-        print("shape", nx, nv, q, tile, fuse, order, flush=True)
-        p.set_tuning(row_q=q, drift_tile=tile, fuse_field=fuse, drift_order=order)
+        print("shape", nx, nv, q, tile, fuse, order, per, flush=True)
+        p.set_tuning(row_q=q, drift_tile=tile, fuse_field=fuse, drift_order=order, drift_tiles=per)
         f = torch.as_tensor(f0, device=dev).clone()
         e = torch.as_tensor(e0, device=dev).clone()
         try:
@@ -83,8 +83,8 @@ def test_autotune_picks_a_valid_shape_and_leaves_state_alone():
     e = torch.as_tensor(e0, device=dev).clone()
     t = p.autotune(n_steps=10)
     print("tuning", t, flush=True)
-    assert t["row_q"] in (4, 8) and t["drift_tile"] in (4, 8, 16, 32) and t["fuse_field"] in (0, 1)
-    assert t["drift_order"] in (0, 1)
+    assert t["row_q"] in (4, 8, 16) and t["drift_tile"] in (4, 8, 16, 32) and t["fuse_field"] in (0, 1)
+    assert t["drift_order"] in (0, 1) and t["drift_tiles"] in (1, 2, 4)
     assert t["us_per_step"] > 0.0
     assert p.tuning() == t
     assert torch.equal(f, torch.as_tensor(f0, dtype=torch.float32, device=dev))
@@ -105,7 +105,7 @@ def test_batch_autotune_keeps_parity():
     print("batch tuning", tun, flush=True)
     assert set(tun) == {(32, 256), (64, 512)}
     for t in tun.values():
-        assert t["row_q"] in (4, 8) and t["drift_tile"] in (4, 8, 16, 32) and t["us_per_step"] > 0.0
+        assert t["row_q"] in (4, 8, 16) and t["drift_tile"] in (4, 8, 16, 32) and t["us_per_step"] > 0.0
     n = 10
     e2 = sc.run(n)
     for i in range(3):

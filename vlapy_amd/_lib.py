@@ -63,12 +63,13 @@ class Tuning(ctypes.Structure):
         ("drift_tile", _c_i),
         ("fuse_field", _c_i),
         ("drift_order", _c_i),
+        ("drift_tiles", _c_i),
         ("us_per_step", _c_d),
     ]
 
     def as_dict(self):
         return {"row_q": self.row_q, "drift_tile": self.drift_tile, "fuse_field": self.fuse_field,
-                "drift_order": self.drift_order, "us_per_step": self.us_per_step}
+                "drift_order": self.drift_order, "drift_tiles": self.drift_tiles, "us_per_step": self.us_per_step}
 
 
 # name -> (argtypes); every function returns int status unless listed in _OTHER
@@ -201,7 +202,7 @@ class Plan:
         check(library().vpfp_plan_get_tuning(self.handle, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, row_q=None, drift_tile=None, fuse_field=None, drift_order=None):
+    def set_tuning(self, row_q=None, drift_tile=None, fuse_field=None, drift_order=None, drift_tiles=None):
         t = Tuning()
         check(library().vpfp_plan_get_tuning(self.handle, ctypes.byref(t)))
         if row_q is not None:
@@ -212,6 +213,8 @@ class Plan:
             t.fuse_field = int(fuse_field)
         if drift_order is not None:
             t.drift_order = int(drift_order)
+        if drift_tiles is not None:
+            t.drift_tiles = int(drift_tiles)
         check(library().vpfp_plan_set_tuning(self.handle, ctypes.byref(t)))
 
     def autotune(self, n_steps=20, stream=None):

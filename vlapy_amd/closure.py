@@ -172,7 +172,7 @@ def downsample(f, factor):
     return f.reshape(n, nx, nv // factor, factor).mean(-1)
 
 
-def training_pairs(inst, n_steps, every, span=1, fine_precision="fp64"):
+def training_pairs(inst, n_steps, every, span=1, fine_precision="fp64", log=None):
     """(input, target) rows from fine runs (nv = 4096) of the instances.
 
     Every `every` steps the fine state is averaged onto the coarse grid,
@@ -200,6 +200,8 @@ def training_pairs(inst, n_steps, every, span=1, fine_precision="fp64"):
         t = (downsample(fc.f, factor).float().reshape(-1, NV) - cc.f.reshape(-1, NV)) / span
         xs.append(s)
         ts.append(t)
+        if log is not None:
+            log(done)
     return torch.cat(xs), torch.cat(ts)
 
 

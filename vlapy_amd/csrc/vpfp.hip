@@ -130,9 +130,14 @@ constexpr int qfor(int M) { return M < 8 ? M : 8; }
 // LDS per workgroup on gfx950
 constexpr int kMaxLds = 160 * 1024;
 
-// drift workgroups: one per column tile up to 1024, then grid-stride
+// drift workgroups: one per `per` column tiles (at most 1024 for a single
+// grid, which then strides); each workgroup writes one partial density row
 constexpr int kDriftMaxWG = 1024;
 int drift_grid(int ntiles) { return ntiles < kDriftMaxWG ? ntiles : kDriftMaxWG; }
+int drift_wgs(int nv, int C, int per, bool batch) {
+  const int tiles = nv / C, w = (tiles + per - 1) / (per > 0 ? per : 1);
+  return batch ? w : drift_grid(w);
+}
 
 // field workgroups: one per 64 x values (the last to finish solves Poisson)
 int field_grid(int nx) { return nx >= 128 ? nx / 64 : 1; }
@@ -150,10 +155,13 @@ int launch_row_q(const vp::RowArgs<T>& a, hipStream_t s) {
 // Q = 8 values per thread fills the GPU when there are many rows; with few
 // rows (fewer than ~16k threads at Q = 8) Q = 4 spreads each row over twice
 // as many threads and shortens the per-row latency chain.
-// q: 4 or 8 forces that many values per thread (where both exist), 0 = the
-// row-count rule above
+// q: 4, 8 or 16 forces that many values per thread (where it exists: 4 for
+// 16 <= M <= 2048, 16 for M >= 256), 0 = the row-count rule above
 template <class T, int M>
 int launch_row(const vp::RowArgs<T>& a, hipStream_t s, int q) {
+  if constexpr (M >= 256 && M <= 4096) {
+    if (q == 16) return launch_row_q<T, M, 16>(a, s);
+  }
   if constexpr (M >= 16 && M <= 2048) {
     if (q == 4 || (q == 0 && (long long)a.nx * (M / 8) < 16384)) return launch_row_q<T, M, 4>(a, s);
   }
@@ -172,7 +180,7 @@ int launch_tri(const vp::TriArgs<T>& a, hipStream_t s) {
 }
 
 template <class T, int M, int C>
-int launch_drift_c(const vp::DriftArgs<T>& a, hipStream_t s, int gy) {
+int launch_drift_c(const vp::DriftArgs<T>& a, hipStream_t s, int gx, int gy) {
   constexpr int Q = M <= 64 ? (M < 4 ? M : 4) : 8;
   using Cfg = vp::DriftCfg<T, M, Q, C>;
   auto kern = k_drift<T, M, Q, C>;
@@ -180,20 +188,20 @@ int launch_drift_c(const vp::DriftArgs<T>& a, hipStream_t s, int gy) {
     return fail(VPFP_EUNSUPPORTED, "drift tile of " + std::to_string(C) + " columns does not fit a workgroup");
   if (a.fuse_field && !Cfg::tail) return fail(VPFP_EUNSUPPORTED, "field tail not available at this size");
   if (int rc = set_lds(kern, Cfg::lds)) return rc;
-  // gy > 0: batch of gy instances, one column tile per workgroup; gy == 0:
-  // one grid, at most drift_grid(ntiles) workgroups striding over the tiles
-  const dim3 grid = gy > 0 ? dim3(a.nv / C, gy) : dim3(drift_grid(a.nv / C));
+  // gx workgroups stride over the nv / C column tiles (drift_wgs); gy > 0:
+  // a batch of gy instances (grid y = instance)
+  const dim3 grid = gy > 0 ? dim3(gx, gy) : dim3(gx);
   hipLaunchKernelGGL(kern, grid, dim3(Cfg::wg), Cfg::lds, s, a);
   return launched();
 }
 
 template <class T, int M>
-int launch_drift(const vp::DriftArgs<T>& a, int C, hipStream_t s, int gy) {
+int launch_drift(const vp::DriftArgs<T>& a, int C, hipStream_t s, int gx, int gy) {
   switch (C) {
-    case 4: return launch_drift_c<T, M, 4>(a, s, gy);
-    case 8: return launch_drift_c<T, M, 8>(a, s, gy);
-    case 16: return launch_drift_c<T, M, 16>(a, s, gy);
-    case 32: return launch_drift_c<T, M, 32>(a, s, gy);
+    case 4: return launch_drift_c<T, M, 4>(a, s, gx, gy);
+    case 8: return launch_drift_c<T, M, 8>(a, s, gx, gy);
+    case 16: return launch_drift_c<T, M, 16>(a, s, gx, gy);
+    case 32: return launch_drift_c<T, M, 32>(a, s, gx, gy);
   }
   return fail(VPFP_EUNSUPPORTED, "drift tile width");
 }
@@ -247,9 +255,9 @@ int tri_dispatch(const vp::TriArgs<T>& a, hipStream_t s) {
 }
 
 template <class T>
-int drift_dispatch(const vp::DriftArgs<T>& a, int C, hipStream_t s, int gy = 0) {
+int drift_dispatch(const vp::DriftArgs<T>& a, int C, int gx, hipStream_t s, int gy = 0) {
   const int M = a.nx / 2;
-#define CALL(m) launch_drift<T, ((m) <= 1024 ? (m) : 2)>(a, C, s, gy)
+#define CALL(m) launch_drift<T, ((m) <= 1024 ? (m) : 2)>(a, C, s, gx, gy)
   VPFP_SWITCH_M(M, CALL, 1024)
 #undef CALL
   return fail(VPFP_EUNSUPPORTED, "nx must be a power of two in [4, 2048] for the x-advection");
@@ -311,6 +319,7 @@ struct vpfp_plan_s {
   int row_q = 0;   // values per thread of the row kernels (0: row-count rule)
   int fuse = -1;   // field solved by the last drift workgroup: 1 yes, 0 no, -1 size rule
   int xcd = 1;     // drift tile order (xcd_block); never slower in the measured shapes
+  int dtiles = 1;  // column tiles per drift workgroup
   double tuned_us = 0.0;
   double *kx = nullptr, *okx = nullptr, *v = nullptr, *kv = nullptr;
   void* tw_v = nullptr;          // cx<T> [lo | hi] tables for nv
@@ -545,7 +554,7 @@ int do_edfdv(vpfp_plan_t p, const void* fin, void* fout, const double* e, double
 
 template <class T>
 int do_vdfdx(vpfp_plan_t p, const void* fin, void* fout, double dt, double* dens, hipStream_t s) {
-  return drift_dispatch<T>(drift_args<T>(p, fin, fout, dt, dens), p->C, s);
+  return drift_dispatch<T>(drift_args<T>(p, fin, fout, dt, dens), p->C, p->ntiles, s);
 }
 
 template <class T>
@@ -606,7 +615,7 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
       auto da = drift_args<T>(p, f, f, p->dt, p->dens);
       da.fuse_field = 1;
       da.fld = fa;
-      PROF(KK_DRIFT, drift_dispatch<T>(da, p->C, s));
+      PROF(KK_DRIFT, drift_dispatch<T>(da, p->C, p->ntiles, s));
     } else {
       PROF(KK_DRIFT, (do_vdfdx<T>(p, f, f, p->dt, p->dens, s)));
       PROF(KK_FIELD, field_dispatch(fa, true, s));
@@ -632,6 +641,7 @@ struct vpfp_batch_s {
   int n = 0, nx = 0, nv = 0, prec = VPFP_F64, op = VPFP_OP_DG, C = 0, gx = 0;
   int row_q = 0;
   int xcd = 1;
+  int dtiles = 1;
   double tuned_us = 0.0;
   vpfc_net_t closure = nullptr;  // learned closure after every step (FP32, nv 256)
   double dv = 0, dt = 0, v0 = 0, dvv = 0, dk_v = 0;
@@ -735,7 +745,7 @@ int batch_step_n(vpfp_batch_s* b, void* f, double* e, int n, double* series, hip
     fa.e_out = ei;
     fa.series = series ? series + (size_t)i * b->n * 2 : nullptr;
     fa.series_stride = 2;
-    if (int rc = drift_dispatch<T>(da, b->C, s, b->n)) return rc;
+    if (int rc = drift_dispatch<T>(da, b->C, b->gx, s, b->n)) return rc;
     if (int rc = field_dispatch(fa, true, s, b->n)) return rc;
     e_cur = ei;
     ra.e = e_cur;
@@ -809,55 +819,93 @@ struct TuneScratch {
 
 std::vector<int> row_q_candidates(int nv) {
   const int M = nv / 2;
-  if (M >= 16 && M <= 2048) return {4, 8};
-  return {8};
+  std::vector<int> q;
+  if (M >= 16 && M <= 2048) q.push_back(4);
+  q.push_back(8);
+  if (M >= 256 && M <= 4096) q.push_back(16);
+  return q;
+}
+
+// the launch shape under test / kept
+struct Shape {
+  int q, C, fuse, xcd, dt;
+};
+
+// Times every candidate shape with run(n) after `apply(shape)`; keeps the
+// fastest (restores `keep` when none ran).  A shape that does not fit the
+// grid returns VPFP_EUNSUPPORTED and is skipped; any other error ends the
+// search and is returned.
+template <class Apply, class Run>
+int tune_search(const std::vector<Shape>& cands, const Shape& keep, Apply apply, Run run, TuneScratch& w,
+                int n_steps, hipStream_t s, double* best_us) {
+  double best = 1e300;
+  Shape bs = keep;
+  for (const Shape& c : cands) {
+    apply(c);
+    double us = 0.0;
+    const int rc = w.time(run, n_steps, s, &us);
+    if (rc == VPFP_EUNSUPPORTED) continue;
+    if (rc) {
+      apply(keep);
+      return rc;
+    }
+    if (us < best) {
+      best = us;
+      bs = c;
+    }
+  }
+  apply(best < 1e300 ? bs : keep);
+  if (!(best < 1e300)) return fail(VPFP_EHIP, "autotune: no launch shape ran");
+  *best_us = best;
+  return VPFP_OK;
+}
+
+void plan_shape(vpfp_plan_t p, const Shape& c) {
+  p->row_q = c.q;
+  p->C = c.C;
+  p->fuse = c.fuse;
+  p->xcd = c.xcd;
+  p->dtiles = c.dt;
+  p->ntiles = drift_wgs(p->nv, p->C, p->dtiles, false);
+}
+
+void batch_shape(vpfp_batch_s* b, const Shape& c) {
+  b->row_q = c.q;
+  b->C = c.C;
+  b->xcd = c.xcd;
+  b->dtiles = c.dt;
+  b->gx = drift_wgs(b->nv, b->C, b->dtiles, true);
 }
 
 template <class T>
 int autotune_plan(vpfp_plan_t p, int n_steps, hipStream_t s) {
   TuneScratch w;
   if (int rc = w.init(tune_state<T>(p->nx, p->nx, p->nv, p->v0, p->dvv), 2 * (size_t)p->nx)) return rc;
-  const int q0 = p->row_q, c0 = p->C, f0 = p->fuse, x0 = p->xcd;
-  const bool prof = p->profiling;
-  p->profiling = false;
-  double best = 1e300;
-  int bq = q0, bc = c0, bf = f0, bx = x0, rc = VPFP_OK;
-  for (int q : row_q_candidates(p->nv)) {
+  std::vector<Shape> cands;
+  for (int q : row_q_candidates(p->nv))
     for (int C : {4, 8, 16, 32}) {
       if (C > p->nv) continue;
       for (int fuse : {0, 1}) {
         if (fuse && p->nx > 256) continue;
-        for (int xo : {0, 1}) {
-          p->row_q = q;
-          p->C = C;
-          p->ntiles = drift_grid(p->nv / C);
-          p->fuse = fuse;
-          p->xcd = xo;
-          double us = 0.0;
-          rc = w.time([&](int n) { return do_step_n<T>(p, w.f, w.e, n, nullptr, nullptr, nullptr, nullptr, s); },
-                      n_steps, s, &us);
-          if (rc == VPFP_EUNSUPPORTED) continue;  // shape does not fit this grid
-          if (rc) break;
-          if (us < best) {
-            best = us;
-            bq = q, bc = C, bf = fuse, bx = xo;
+        for (int xo : {0, 1})
+          for (int dt : {1, 2, 4}) {
+            // more tiles per workgroup only where it changes the grid
+            if (dt > 1 && p->nv / C / dt < 8) continue;
+            cands.push_back({q, C, fuse, xo, dt});
           }
-        }
-        if (rc && rc != VPFP_EUNSUPPORTED) break;
       }
-      if (rc && rc != VPFP_EUNSUPPORTED) break;
     }
-    if (rc && rc != VPFP_EUNSUPPORTED) break;
-  }
+  const Shape keep{p->row_q, p->C, p->fuse, p->xcd, p->dtiles};
+  const bool prof = p->profiling;
+  p->profiling = false;
+  double us = 0.0;
+  const int rc = tune_search(
+      cands, keep, [&](const Shape& c) { plan_shape(p, c); },
+      [&](int n) { return do_step_n<T>(p, w.f, w.e, n, nullptr, nullptr, nullptr, nullptr, s); }, w, n_steps, s,
+      &us);
   p->profiling = prof;
-  const bool ok = !(rc && rc != VPFP_EUNSUPPORTED) && best < 1e300;
-  p->row_q = ok ? bq : q0;
-  p->C = ok ? bc : c0;
-  p->ntiles = drift_grid(p->nv / p->C);
-  p->fuse = ok ? bf : f0;
-  p->xcd = ok ? bx : x0;
-  if (!ok) return rc ? rc : fail(VPFP_EHIP, "autotune: no launch shape ran");
-  p->tuned_us = best;
+  if (rc) return rc;
+  p->tuned_us = us;
   return VPFP_OK;
 }
 
@@ -866,43 +914,31 @@ int autotune_batch(vpfp_batch_s* b, int n_steps, hipStream_t s) {
   TuneScratch w;
   const size_t rows = (size_t)b->n * b->nx;
   if (int rc = w.init(tune_state<T>(rows, b->nx, b->nv, b->v0, b->dvv), rows)) return rc;
-  const int q0 = b->row_q, c0 = b->C, x0 = b->xcd;
-  double best = 1e300;
-  int bq = q0, bc = c0, bx = x0, rc = VPFP_OK;
-  for (int q : row_q_candidates(b->nv)) {
+  std::vector<Shape> cands;
+  for (int q : row_q_candidates(b->nv))
     for (int C : {4, 8, 16, 32}) {
       if (C > b->nv) continue;
-      for (int xo : {0, 1}) {
-        b->row_q = q;
-        b->C = C;
-        b->gx = b->nv / C;
-        b->xcd = xo;
-        double us = 0.0;
-        rc = w.time([&](int n) { return batch_step_n<T>(b, w.f, w.e, n, nullptr, s); }, n_steps, s, &us);
-        if (rc == VPFP_EUNSUPPORTED) continue;
-        if (rc) break;
-        if (us < best) {
-          best = us;
-          bq = q, bc = C, bx = xo;
+      for (int xo : {0, 1})
+        for (int dt : {1, 2, 4}) {
+          if (dt > 1 && b->nv / C / dt < 2) continue;
+          cands.push_back({q, C, 0, xo, dt});
         }
-      }
-      if (rc && rc != VPFP_EUNSUPPORTED) break;
     }
-    if (rc && rc != VPFP_EUNSUPPORTED) break;
-  }
-  const bool ok = !(rc && rc != VPFP_EUNSUPPORTED) && best < 1e300;
-  b->row_q = ok ? bq : q0;
-  b->C = ok ? bc : c0;
-  b->gx = b->nv / b->C;
-  b->xcd = ok ? bx : x0;
-  if (!ok) return rc ? rc : fail(VPFP_EHIP, "autotune: no launch shape ran");
-  b->tuned_us = best;
+  const Shape keep{b->row_q, b->C, 0, b->xcd, b->dtiles};
+  double us = 0.0;
+  const int rc = tune_search(
+      cands, keep, [&](const Shape& c) { batch_shape(b, c); },
+      [&](int n) { return batch_step_n<T>(b, w.f, w.e, n, nullptr, s); }, w, n_steps, s, &us);
+  if (rc) return rc;
+  b->tuned_us = us;
   return VPFP_OK;
 }
 
 int check_tuning(const vpfp_tuning* t, int nv) {
   if (!t) return fail(VPFP_EBADARG, "null tuning");
-  if (t->row_q != 0 && t->row_q != 4 && t->row_q != 8) return fail(VPFP_EBADARG, "row_q must be 0, 4 or 8");
+  if (t->row_q != 0 && t->row_q != 4 && t->row_q != 8 && t->row_q != 16)
+    return fail(VPFP_EBADARG, "row_q must be 0, 4, 8 or 16");
+  if (t->drift_tiles < 0 || t->drift_tiles > 64) return fail(VPFP_EBADARG, "drift_tiles must be in [0, 64]");
   if (t->drift_tile != 0 && t->drift_tile != 4 && t->drift_tile != 8 && t->drift_tile != 16 && t->drift_tile != 32)
     return fail(VPFP_EBADARG, "drift_tile must be 0, 4, 8, 16 or 32");
   if (t->drift_order != 0 && t->drift_order != 1) return fail(VPFP_EBADARG, "drift_order must be 0 or 1");
@@ -950,7 +986,7 @@ int vpfp_plan_create(const vpfp_grid_desc* d, vpfp_plan_t* out) {
   p->nx_pow2 = is_pow2(d->nx);
   p->nv_pow2 = is_pow2(d->nv);
   p->C = (p->nx_pow2 && p->nv_pow2 && p->nx >= 4) ? drift_tile(p->nx, p->nv) : 0;
-  p->ntiles = p->C ? drift_grid(p->nv / p->C) : 0;
+  p->ntiles = p->C ? drift_wgs(p->nv, p->C, p->dtiles, false) : 0;
   int rc = VPFP_OK;
   const size_t nxb = sizeof(double) * d->nx, nvb = sizeof(double) * d->nv;
   if (!rc) rc = dalloc(p, &p->kx, nxb);
@@ -1126,7 +1162,7 @@ int vpfp_batch_create(const vpfp_batch_desc* d, vpfp_batch_t* out) {
   b->dvv = dvv;
   b->dk_v = d->kv[1];
   b->C = drift_tile(d->nx, d->nv);
-  b->gx = d->nv / b->C;
+  b->gx = drift_wgs(d->nv, b->C, b->dtiles, true);
   std::vector<double> vv(d->v, d->v + d->nv);
   std::vector<double> okx(d->one_over_kx, d->one_over_kx + (size_t)d->n_inst * d->nx);
   int rc = VPFP_OK;
@@ -1181,6 +1217,7 @@ int vpfp_plan_get_tuning(vpfp_plan_t p, vpfp_tuning* t) {
   t->drift_tile = p->C;
   t->fuse_field = p->fuse;
   t->drift_order = p->xcd;
+  t->drift_tiles = p->dtiles;
   t->us_per_step = p->tuned_us;
   return VPFP_OK;
 }
@@ -1189,10 +1226,9 @@ int vpfp_plan_set_tuning(vpfp_plan_t p, const vpfp_tuning* t) {
   if (int rc = check_plan(p)) return rc;
   if (int rc = check_tuning(t, p->nv)) return rc;
   p->row_q = t->row_q;
-  if (p->C && t->drift_tile) {
-    p->C = t->drift_tile;
-    p->ntiles = drift_grid(p->nv / p->C);
-  }
+  if (t->drift_tiles) p->dtiles = t->drift_tiles;
+  if (p->C && t->drift_tile) p->C = t->drift_tile;
+  if (p->C) p->ntiles = drift_wgs(p->nv, p->C, p->dtiles, false);
   p->fuse = t->fuse_field;
   p->xcd = t->drift_order;
   return VPFP_OK;
@@ -1216,6 +1252,7 @@ int vpfp_batch_get_tuning(vpfp_batch_t b, vpfp_tuning* t) {
   t->drift_tile = b->C;
   t->fuse_field = 0;
   t->drift_order = b->xcd;
+  t->drift_tiles = b->dtiles;
   t->us_per_step = b->tuned_us;
   return VPFP_OK;
 }
@@ -1224,10 +1261,9 @@ int vpfp_batch_set_tuning(vpfp_batch_t b, const vpfp_tuning* t) {
   if (!b) return fail(VPFP_EBADARG, "null batch");
   if (int rc = check_tuning(t, b->nv)) return rc;
   b->row_q = t->row_q;
-  if (t->drift_tile) {
-    b->C = t->drift_tile;
-    b->gx = b->nv / b->C;
-  }
+  if (t->drift_tiles) b->dtiles = t->drift_tiles;
+  if (t->drift_tile) b->C = t->drift_tile;
+  b->gx = drift_wgs(b->nv, b->C, b->dtiles, true);
   b->xcd = t->drift_order;
   return VPFP_OK;
 }
