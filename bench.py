@@ -389,7 +389,8 @@ def large_section(steps, autotune=True):
         f = torch.as_tensor(f0, dtype=fdt, device=dev).contiguous()
         e = torch.zeros(LG_NX, dtype=torch.float64, device=dev)
         eh = torch.zeros(steps, LG_NX, dtype=torch.float64, device=dev)
-        ops.step_n(plan, f, e, 5, None, eh[:5])
+        warm, prof_n = min(5, steps), min(20, steps)
+        ops.step_n(plan, f, e, warm, None, eh[:warm])
         f = torch.as_tensor(f0, dtype=fdt, device=dev).contiguous()
         e.zero_()
         torch.cuda.synchronize()
@@ -402,14 +403,66 @@ def large_section(steps, autotune=True):
         if not torch.isfinite(eh).all():
             raise RuntimeError("non-finite field history (large grid)")
         plan.set_profiling(True)
-        ops.step_n(plan, f, e, 20, None, eh[:20])
+        ops.step_n(plan, f, e, prof_n, None, eh[:prof_n])
         torch.cuda.synchronize()
         run = {"kernels": plan.kernel_times(), "bytes_f": 8 if prec == "fp64" else 4,
                "ntiles": min(LG_NV // tuning["drift_tile"], 1024), "nx": LG_NX}
         plan.set_profiling(False)
         out[prec] = {"cell_updates_per_s": cells / (ms / 1e3), "ms_per_step": ms, "tuning": tuning,
-                     "roofline": roofline(run, cells, ms)}
+                     "roofline": roofline(run, cells, ms, "traffic_large_%s.json" % prec)}
         del plan, f
+    return out
+
+
+# --------------------------------------------------------------------------- closure
+
+CLOSURE_N, CLOSURE_SEED, CLOSURE_STEPS = 512, 12, 4000
+
+
+def closure_section():
+    """The learned-closure configuration: CLOSURE_N held-out runs (nx=64, dt=0.05,
+    4000 steps, parameters from vlapy_amd.closure.draw_instances) on the fine grid
+    (nv=4096, FP64: the plain FP64 path), the coarse grid (nv=256, FP32) and the
+    coarse grid with the trained closure; instances/s and errors against the fine
+    grid."""
+    import numpy as np
+    import torch
+
+    from vlapy_amd import closure as CL
+
+    path = os.path.join(HERE, "vlapy_amd", "data", "closure_c5.npz")
+    if not os.path.exists(path):
+        return None
+    net = CL.Closure.load(path, max_rows=64)
+    inst = CL.draw_instances(CLOSURE_N, CLOSURE_SEED)
+    t = CL.DT * (np.arange(CLOSURE_STEPS) + 1)
+    out = {"workload": "%d runs, k ~ U[0.25, 0.45], amplitude 10^U[-4,-1], nu 10^U[-5,-2] (Dougherty), nx=64, "
+                       "dt=0.05, %d steps; fine grid nv=4096 FP64 vs coarse nv=256 FP32 with and without the "
+                       "closure" % (CLOSURE_N, CLOSURE_STEPS)}
+    runs = (("fine_fp64", CL.NV_FINE, "fp64", None), ("coarse_fp32", CL.NV, "fp32", None),
+            ("coarse_fp32_closure", CL.NV, "fp32", net))
+    ref = None
+    for name, nv, prec, n in runs:
+        CL.rollout(CL.draw_instances(4, 99), nv, prec, 2, n)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e2 = CL.rollout(inst, nv, prec, CLOSURE_STEPS, n)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        r = {"instances_per_s": CLOSURE_N / sec, "seconds": sec,
+             "density_drift_max": float(np.max(CL.rollout.density_drift))}
+        if ref is None:
+            ref = e2
+        else:
+            h = CL.history_errors(ref, e2)
+            d = CL.damping_errors(inst, ref, e2, t)
+            ok = ~np.isnan(d)
+            r.update({"history_err_median": float(np.median(h)), "damping_fitted": int(ok.sum()),
+                      "damping_err_median": float(np.median(d[ok])),
+                      "damping_within_1pct": float(np.mean(d[ok] <= 0.01))})
+        out[name] = r
+    out["speedup_vs_fine_fp64"] = {k: out[k]["instances_per_s"] / out["fine_fp64"]["instances_per_s"]
+                                   for k in ("coarse_fp32", "coarse_fp32_closure")}
     return out
 
 
@@ -425,6 +478,7 @@ def main():
     ap.add_argument("--no-scan", action="store_true")
     ap.add_argument("--large-steps", type=int, default=200)
     ap.add_argument("--no-large", action="store_true")
+    ap.add_argument("--no-closure", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (for counter runs that should see only the timed shapes)")
     ap.add_argument("--copy-bytes", type=int, default=None,
@@ -500,6 +554,8 @@ def main():
         out["scan"] = scan_section(args.scan_steps, not args.no_cpu)
     if not args.no_large:
         out["large_grid"] = large_section(args.large_steps, at)
+    if not args.no_closure:
+        out["closure"] = closure_section()
     print(json.dumps(out))
     return 0
 

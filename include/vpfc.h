@@ -55,13 +55,30 @@ int vpfc_set_params(vpfc_net_t net, const float* host_params);
 int vpfc_get_params(vpfc_net_t net, float* host_params);
 int vpfc_set_scales(vpfc_net_t net, float in_scale, float out_scale);
 
-/* y = net(in_scale * x) for `rows` rows */
+/* Background profile bg [256] (host, FP32; NULL clears it).  With a
+ * background every row is seen relative to it and to its own size:
+ *     d = f - bg,  s = sqrt(mean_v d^2 + 1e-30),  input in_scale * d / s,
+ * the correction applied is out_scale * s * net(...) and the training target
+ * is compared in units of s (target / s).  Without one, d = f and s = 1. */
+int vpfc_set_background(vpfc_net_t net, const float* host_bg);
+
+/* Conservation projection of the correction (host arrays v [256], bg
+ * [256], FP64; v == NULL clears it): the network output y is replaced by
+ *     y - bg(v) (c0 + c1 v + c2 v^2)
+ * with c chosen so that the trapz moments sum_v w_v v^l y (l = 0, 1, 2;
+ * w = 1, ends 1/2) vanish: the closure then adds no density, momentum or
+ * kinetic energy to a row (to FP32 rounding).  Applied in vpfc_forward and
+ * vpfc_apply; vpfc_gradients and vpfc_train_step see the unprojected
+ * output (train on projected targets). */
+int vpfc_set_conservation(vpfc_net_t net, const double* v, const double* bg);
+
+/* y = net(in_scale * d / s) for `rows` rows (d, s as above) */
 int vpfc_forward(vpfc_net_t net, const float* x, int rows, float* y, void* stream);
 
-/* f <- f + out_scale * net(in_scale * f), in place, `rows` rows */
+/* f <- f + out_scale * s * net(in_scale * d / s), in place, `rows` rows */
 int vpfc_apply(vpfc_net_t net, float* f, int rows, void* stream);
 
-/* Loss L = mean over rows and v of (net(in_scale * x) - target)^2 and its
+/* Loss L = mean over rows and v of (net(in_scale * d / s) - target / s)^2 and its
  * gradient with respect to the flat parameters: grad [VPFC_NPARAMS] and
  * loss [1] are device arrays.  rows <= max_rows. */
 int vpfc_gradients(vpfc_net_t net, const float* x, const float* target, int rows, float* grad, float* loss,

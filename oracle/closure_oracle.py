@@ -9,6 +9,8 @@ in include/vpfc.h:
 
     net: conv1d over v (kernel 5, zero padding 2), 1 -> 64 -> 64 -> 64 -> 1,
          tanh after the first three layers, linear output
+    input in_scale * d / s with d = f - bg, s = rms_v d per row (with a
+    background bg; d = f, s = 1 without one); target compared as target / s
     out[co][v] = b[co] + sum_{k, ci} W[co][k*64 + ci] in[ci][v + k - 2]
 
 With bf16=True every factor of every product is rounded to BF16 (round to
@@ -87,11 +89,26 @@ def _wgrad(dz, x):
     return np.stack([np.einsum("rov,rcv->oc", dz, xp[:, :, k:k + NV], optimize=True) for k in range(KS)], axis=1)
 
 
-def forward(p, f, in_scale, bf16=True):
-    """y = net(in_scale * f) for rows f [R, 256]; returns y and the cache."""
+def row_scale(f, bg):
+    """d = f - bg and s = sqrt(mean_v d^2 + 1e-30) per row (s = 1 without bg)."""
+    f = np.asarray(f, dtype=np.float64)
+    if bg is None:
+        return f, np.ones((f.shape[0], 1))
+    d = f - np.asarray(bg, dtype=np.float64)[None, :]
+    return d, np.sqrt(np.mean(d * d, axis=1, keepdims=True) + 1e-30)
+
+
+def forward(p, f, in_scale, bf16=True, bg=None):
+    """y = net(in_scale * d / s) for rows f [R, 256] (d, s from row_scale);
+    returns y and the cache."""
     r = bf16_round if bf16 else (lambda a: np.asarray(a, dtype=np.float64))
     q = unpack(p)
-    x = r(np.float32(in_scale) * np.asarray(f, dtype=np.float32)) if bf16 else in_scale * np.asarray(f, np.float64)
+    d, s = row_scale(f, bg)
+    if bf16:
+        # FP32 as the GPU forms it: (in_scale / s) * d
+        x = r(np.asarray((np.float32(in_scale) / s.astype(np.float32)) * d.astype(np.float32), dtype=np.float32))
+    else:
+        x = in_scale * d / s
     x = np.asarray(x, dtype=np.float64)
     W1 = np.asarray(r(q["W1"]), np.float64)
     z1 = np.zeros((x.shape[0], CH, NV))
@@ -106,16 +123,16 @@ def forward(p, f, in_scale, bf16=True):
     y = np.full((x.shape[0], NV), float(q["b4"][0]))
     for k in range(KS):
         y += np.einsum("c,rcv->rv", W4[k], a3p[:, :, k:k + NV], optimize=True)
-    return y, {"x": x, "a1": a1, "a2": a2, "a3": a3}
+    return y, {"x": x, "a1": a1, "a2": a2, "a3": a3, "s": s}
 
 
-def loss_and_grad(p, f, target, in_scale, bf16=True):
-    """L = mean (y - target)^2 and dL/dp (flat, float64)."""
+def loss_and_grad(p, f, target, in_scale, bf16=True, bg=None):
+    """L = mean (y - target / s)^2 and dL/dp (flat, float64); s = 1 without bg."""
     r = bf16_round if bf16 else (lambda a: np.asarray(a, dtype=np.float64))
     rr = (lambda a: np.asarray(bf16_round(a), np.float64)) if bf16 else (lambda a: np.asarray(a, np.float64))
     q = unpack(p)
-    y, c = forward(p, f, in_scale, bf16)
-    t = np.asarray(target, dtype=np.float64)
+    y, c = forward(p, f, in_scale, bf16, bg)
+    t = np.asarray(target, dtype=np.float64) / c["s"]
     count = y.size
     d = y - t
     loss = float(np.sum(d * d) / count)

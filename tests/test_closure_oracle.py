@@ -3,7 +3,8 @@
 * BF16 rounding equals torch's float32 -> bfloat16 conversion (round to
   nearest even) on random values, ties and subnormals.
 * The backward pass of the oracle (float64, no rounding) equals central
-  finite differences of the loss for parameters of every layer.
+  finite differences of the loss for parameters of every layer, with and
+  without the background-relative row scaling.
 * Adam restates Kingma & Ba (2015): the first step moves every parameter by
   lr * g / (|g| + eps) and repeated steps on a quadratic converge.
 * The BF16 mode differs from the exact mode only by BF16 rounding.
@@ -37,9 +38,12 @@ def _problem(rows=2, seed=1):
     return p, f, t
 
 
-def test_backward_equals_finite_differences():
+@pytest.mark.parametrize("with_bg", [False, True])
+def test_backward_equals_finite_differences(with_bg):
     p, f, t = _problem()
-    loss, g = C.loss_and_grad(p, f, t, 2.5, bf16=False)
+    v = np.linspace(-6.4 + 0.025, 6.4 - 0.025, 256)
+    bg = np.exp(-v ** 2 / 2) / np.sqrt(2 * np.pi) if with_bg else None
+    loss, g = C.loss_and_grad(p, f, t, 2.5, bf16=False, bg=bg)
     rng = np.random.default_rng(7)
     picks = []
     for name in ("W1", "b1", "W2", "b2", "W3", "b3", "W4", "b4"):
@@ -50,8 +54,8 @@ def test_backward_equals_finite_differences():
         pp, pm = p.copy(), p.copy()
         pp[i] += h
         pm[i] -= h
-        lp, _ = C.loss_and_grad(pp, f, t, 2.5, bf16=False)
-        lm, _ = C.loss_and_grad(pm, f, t, 2.5, bf16=False)
+        lp, _ = C.loss_and_grad(pp, f, t, 2.5, bf16=False, bg=bg)
+        lm, _ = C.loss_and_grad(pm, f, t, 2.5, bf16=False, bg=bg)
         fd = (lp - lm) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(abs(fd), 1e-3 * np.abs(g).max()), (i, fd, g[i])
 

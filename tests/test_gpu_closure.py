@@ -20,47 +20,52 @@ from oracle import closure_oracle as C
 pytestmark = pytest.mark.gpu
 
 
-def _setup(rows, seed):
+def _setup(rows, seed, with_bg=False):
     import torch
-    from vlapy_amd.closure import Closure, init_params
+    from vlapy_amd.closure import Closure, background, init_params
 
     rng = np.random.default_rng(seed)
     p = init_params(seed, out_gain=1.0)
     p += (0.05 * rng.standard_normal(p.size)).astype(np.float32)
     v = np.linspace(-6.4 + 0.025, 6.4 - 0.025, 256)
-    f = (np.exp(-v ** 2 / 2)[None, :] / np.sqrt(2 * np.pi) * (1 + 0.3 * rng.standard_normal((rows, 256))))
+    bg = background() if with_bg else None
+    # rows of different perturbation sizes around the Maxwellian
+    amp = 10.0 ** rng.uniform(-4, -1, (rows, 1))
+    f = (np.exp(-v ** 2 / 2)[None, :] / np.sqrt(2 * np.pi) * (1 + amp * rng.standard_normal((rows, 256))))
     f = f.astype(np.float32)
-    t = (0.1 * rng.standard_normal((rows, 256))).astype(np.float32)
-    net = Closure(p, in_scale=2.5, out_scale=0.01, max_rows=max(rows, 8))
+    t = (0.1 * amp * rng.standard_normal((rows, 256))).astype(np.float32)
+    net = Closure(p, in_scale=2.5 if not with_bg else 0.3, out_scale=0.01, max_rows=max(rows, 8), bg=bg)
     dev = torch.device("cuda:0")
-    return net, p, f, t, torch.as_tensor(f, device=dev), torch.as_tensor(t, device=dev)
+    return net, p, f, t, torch.as_tensor(f, device=dev), torch.as_tensor(t, device=dev), bg
 
 
-@pytest.mark.parametrize("rows", [1, 7, 300])
-def test_forward_matches_oracle(rows):
-    net, p, f, t, fd, td = _setup(rows, 3)
-    print("closure forward rows", rows, flush=True)
+@pytest.mark.parametrize("rows,with_bg", [(1, False), (7, False), (300, False), (7, True), (300, True)])
+def test_forward_matches_oracle(rows, with_bg):
+    net, p, f, t, fd, td, bg = _setup(rows, 3, with_bg)
+    print("closure forward rows", rows, "background", with_bg, flush=True)
     y = net.forward(fd).cpu().numpy()
-    y_ref, _ = C.forward(p, f, 2.5, bf16=True)
+    insc = 2.5 if not with_bg else 0.3
+    y_ref, cache = C.forward(p, f, insc, bf16=True, bg=bg)
     scale = np.abs(y_ref).max()
     err = np.abs(y - y_ref)
     l2 = np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref)
     frac = np.mean(err > 1e-4 * scale)
     print("forward rel l2", l2, "max", err.max() / scale, "fraction > 1e-4", frac, flush=True)
     assert l2 <= 2e-3 and frac <= 0.05 and err.max() <= 1e-2 * scale
-    # apply: f + out_scale * y, in place
+    # apply: f + out_scale * s * y, in place
     g = fd.clone()
     net.apply(g)
-    np.testing.assert_allclose(g.cpu().numpy(), f + np.float32(0.01) * y, rtol=0, atol=1e-6)
+    s = cache["s"].astype(np.float32)
+    np.testing.assert_allclose(g.cpu().numpy(), f + np.float32(0.01) * s * y, rtol=0, atol=1e-6)
 
 
-@pytest.mark.parametrize("rows", [2, 33])
-def test_gradients_match_oracle(rows):
-    net, p, f, t, fd, td = _setup(rows, 4)
-    print("closure gradients rows", rows, flush=True)
+@pytest.mark.parametrize("rows,with_bg", [(2, False), (33, False), (33, True)])
+def test_gradients_match_oracle(rows, with_bg):
+    net, p, f, t, fd, td, bg = _setup(rows, 4, with_bg)
+    print("closure gradients rows", rows, "background", with_bg, flush=True)
     g, loss = net.gradients(fd, td)
     g = g.cpu().numpy().astype(np.float64)
-    l_ref, g_ref = C.loss_and_grad(p, f, t, 2.5, bf16=True)
+    l_ref, g_ref = C.loss_and_grad(p, f, t, 2.5 if not with_bg else 0.3, bf16=True, bg=bg)
     assert abs(loss.item() / l_ref - 1) <= 1e-4
     for name in ("W1", "b1", "W2", "b2", "W3", "b3", "W4", "b4"):
         lo, hi = C.OFF[name]
@@ -70,7 +75,7 @@ def test_gradients_match_oracle(rows):
 
 
 def test_adam_step_matches_oracle_and_training_lowers_loss():
-    net, p, f, t, fd, td = _setup(16, 5)
+    net, p, f, t, fd, td, _ = _setup(16, 5)
     g, _ = net.gradients(fd, td)
     g = g.cpu().numpy().astype(np.float64)
     net.train_step(fd, td, lr=1e-3)
@@ -89,3 +94,33 @@ def test_adam_step_matches_oracle_and_training_lowers_loss():
     _, l1 = net.gradients(fd, tgt)
     print("loss", l0.item(), "->", l1.item(), flush=True)
     assert l1.item() < 0.25 * l0.item()
+
+
+def test_conservation_projection():
+    """With the projection on, the correction added by apply() has zero trapz
+    moments 1, v, v^2 (density, momentum, kinetic energy) to FP32 rounding and
+    equals the host projector applied to the unprojected output."""
+    from vlapy_amd.closure import background, conservation_projector, coarse_v
+
+    net, p, f, t, fd, td, bg = _setup(40, 6, with_bg=True)
+    y_raw = net.forward(fd).cpu().numpy().astype(np.float64)
+    v, b = coarse_v(), np.asarray(background(), np.float64)
+    net.set_conservation(v, b)
+    y = net.forward(fd).cpu().numpy().astype(np.float64)
+    P = conservation_projector(v, b)
+    np.testing.assert_allclose(y, y_raw @ P.T, rtol=0, atol=2e-5 * np.abs(y_raw).max())
+    w = np.ones(256)
+    w[0] = w[-1] = 0.5
+    for k in range(3):
+        mom = (y * w * v ** k).sum(axis=1)
+        scale = (np.abs(y) * w * np.abs(v) ** k).sum(axis=1)
+        print("moment", k, np.abs(mom / scale).max(), flush=True)
+        assert np.all(np.abs(mom) <= 1e-5 * scale), k
+    # apply adds out_scale * s * (projected y)
+    from oracle import closure_oracle as C
+
+    _, s = C.row_scale(f, bg)
+    g = fd.clone()
+    net.apply(g)
+    np.testing.assert_allclose(g.cpu().numpy(), f + np.float32(0.01) * s.astype(np.float32) * y.astype(np.float32),
+                               rtol=0, atol=1e-6)

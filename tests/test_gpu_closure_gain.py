@@ -1,0 +1,52 @@
+"""What the trained closure (vlapy_amd/data/closure_c5.npz, written by
+`python -m vlapy_amd.train_closure --relative 0 --conserve 1`) does against the
+method without it, judged by the fine grid.
+
+32 held-out runs (draw_instances(32, seed=12): k ~ U[0.25, 0.45], amplitude
+10^U[-4, -1], nu 10^U[-5, -2]; nx = 64, dt = 0.05, 4000 steps) on the fine
+grid (nv = 4096, FP64), on the coarse grid (nv = 256, FP32) and on the coarse
+grid with the closure after every step.  Printed: median relative error of
+log10 <E^2>(t) and of the damping rate against the fine grid, and the
+fraction of runs within 1 % of the fine damping rate, for both coarse runs.
+Asserted: the closure run stays finite, adds no density beyond the plain
+coarse run's own FP32 drift, and its errors stay within 1.5x of the plain
+coarse run's (the trained closure of this configuration does not improve on
+the plain coarse grid; the numbers printed are the measurement).
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "vlapy_amd", "data",
+                       "closure_c5.npz")
+
+
+def test_closure_against_plain_coarse_grid():
+    from vlapy_amd import closure as CL
+
+    net = CL.Closure.load(WEIGHTS, max_rows=64)
+    inst = CL.draw_instances(32, 12)
+    steps = 4000
+    t = CL.DT * (np.arange(steps) + 1)
+    print("closure gain: fine FP64 rollout", flush=True)
+    ref = CL.rollout(inst, CL.NV_FINE, "fp64", steps)
+    plain = CL.rollout(inst, CL.NV, "fp32", steps)
+    drift_plain = CL.rollout.density_drift.max()
+    clo = CL.rollout(inst, CL.NV, "fp32", steps, net)
+    drift_clo = CL.rollout.density_drift.max()
+    assert np.all(np.isfinite(clo))
+    res = {}
+    for name, e2 in (("plain", plain), ("closure", clo)):
+        h = CL.history_errors(ref, e2)
+        d = CL.damping_errors(inst, ref, e2, t)
+        ok = ~np.isnan(d)
+        res[name] = (np.median(h), np.median(d[ok]), np.mean(d[ok] <= 0.01))
+        print(name, "history err median %.4f damping err median %.5f within 1%% %.3f" % res[name], flush=True)
+    print("density drift max: plain %.2e closure %.2e" % (drift_plain, drift_clo), flush=True)
+    assert drift_clo <= max(2 * drift_plain, 1e-6)
+    assert res["closure"][0] <= 1.5 * res["plain"][0]
+    assert res["closure"][1] <= 1.5 * res["plain"][1] + 1e-3

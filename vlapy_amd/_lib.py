@@ -106,6 +106,8 @@ SIGNATURES = {
     "vpfc_set_params": [_c_p, _c_p],
     "vpfc_get_params": [_c_p, _c_p],
     "vpfc_set_scales": [_c_p, ctypes.c_float, ctypes.c_float],
+    "vpfc_set_background": [_c_p, _c_p],
+    "vpfc_set_conservation": [_c_p, _c_p, _c_p],
     "vpfc_forward": [_c_p, _c_p, _c_i, _c_p, _c_p],
     "vpfc_apply": [_c_p, _c_p, _c_i, _c_p],
     "vpfc_gradients": [_c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p],

@@ -48,7 +48,7 @@ class Closure:
     """One network on the GPU (parameters, Adam moments, training workspace
     for up to max_rows rows)."""
 
-    def __init__(self, params=None, in_scale=1.0, out_scale=1.0, max_rows=4096, seed=0):
+    def __init__(self, params=None, in_scale=1.0, out_scale=1.0, max_rows=4096, seed=0, bg=None):
         lib = _lib.library()
         h = ctypes.c_void_p()
         _lib.check(lib.vpfc_create(int(max_rows), ctypes.byref(h)), closure=True)
@@ -56,6 +56,8 @@ class Closure:
         self.max_rows = int(max_rows)
         self.set_params(init_params(seed) if params is None else params)
         self.set_scales(in_scale, out_scale)
+        self.set_background(bg)
+        self.cons = None
 
     # parameters ---------------------------------------------------------
     def set_params(self, p):
@@ -73,14 +75,48 @@ class Closure:
         self.in_scale, self.out_scale = float(in_scale), float(out_scale)
         _lib.check(_lib.library().vpfc_set_scales(self.handle, self.in_scale, self.out_scale), closure=True)
 
+    def set_background(self, bg):
+        """Background profile [256] (see include/vpfc.h) or None."""
+        if bg is None:
+            self.bg = None
+            _lib.check(_lib.library().vpfc_set_background(self.handle, None), closure=True)
+            return
+        self.bg = np.ascontiguousarray(np.asarray(bg, dtype=np.float32))
+        if self.bg.shape != (NV,):
+            raise ValueError("background must hold %d values" % NV)
+        _lib.check(_lib.library().vpfc_set_background(self.handle, self.bg.ctypes.data_as(ctypes.c_void_p)),
+                   closure=True)
+
+    def set_conservation(self, v, bg):
+        """Project every correction so that it adds no density, momentum or
+        kinetic energy (include/vpfc.h); v None switches it off."""
+        if v is None:
+            self.cons = None
+            _lib.check(_lib.library().vpfc_set_conservation(self.handle, None, None), closure=True)
+            return
+        v = np.ascontiguousarray(np.asarray(v, dtype=np.float64))
+        bg = np.ascontiguousarray(np.asarray(bg, dtype=np.float64))
+        if v.shape != (NV,) or bg.shape != (NV,):
+            raise ValueError("v and bg must hold %d values" % NV)
+        self.cons = (v, bg)
+        _lib.check(_lib.library().vpfc_set_conservation(self.handle, v.ctypes.data_as(ctypes.c_void_p),
+                                                        bg.ctypes.data_as(ctypes.c_void_p)), closure=True)
+
     def save(self, path, **meta):
-        np.savez(path, params=self.params(), in_scale=self.in_scale, out_scale=self.out_scale,
+        extra = {} if self.bg is None else {"bg": self.bg}
+        if self.cons is not None:
+            extra.update(cons_v=self.cons[0], cons_bg=self.cons[1])
+        np.savez(path, params=self.params(), in_scale=self.in_scale, out_scale=self.out_scale, **extra,
                  **{k: np.asarray(v) for k, v in meta.items()})
 
     @classmethod
     def load(cls, path, max_rows=4096):
         with np.load(path, allow_pickle=False) as z:
-            return cls(z["params"], float(z["in_scale"]), float(z["out_scale"]), max_rows=max_rows)
+            bg = z["bg"] if "bg" in z.files else None
+            net = cls(z["params"], float(z["in_scale"]), float(z["out_scale"]), max_rows=max_rows, bg=bg)
+            if "cons_v" in z.files:
+                net.set_conservation(z["cons_v"], z["cons_bg"])
+            return net
 
     # compute ------------------------------------------------------------
     @staticmethod
@@ -165,6 +201,32 @@ def _scan(inst, nv, precision):
     return Scan(p, vmax=VMAX, dt=DT, op="dg", precision=precision)
 
 
+def background():
+    """The unit Maxwellian of the fine grid averaged onto the coarse grid
+    (FP32 [256]): the coarse representation of the equilibrium."""
+    dvf = 2 * VMAX / NV_FINE
+    vf = np.linspace(-VMAX + dvf / 2.0, VMAX - dvf / 2.0, NV_FINE)
+    m = np.exp(-vf ** 2 / 2.0) / np.sqrt(2.0 * np.pi)
+    return m.reshape(NV, NV_FINE // NV).mean(-1).astype(np.float32)
+
+
+def coarse_v():
+    """Cell centres of the coarse velocity grid on [-VMAX, VMAX]."""
+    dv = 2 * VMAX / NV
+    return np.linspace(-VMAX + dv / 2.0, VMAX - dv / 2.0, NV)
+
+
+def conservation_projector(v, bg):
+    """P [256, 256] (float64): P y = y - bg (c0 + c1 v + c2 v^2) with the
+    trapz moments of order 0, 1, 2 of P y zero - the map the kernel applies
+    to every correction (vpfc_set_conservation)."""
+    w = np.ones(NV)
+    w[0] = w[-1] = 0.5
+    B = np.stack([bg, v * bg, v * v * bg], 1)
+    Vm = np.stack([w, w * v, w * v * v], 0)
+    return np.eye(NV) - B @ np.linalg.solve(Vm @ B, Vm)
+
+
 def downsample(f, factor):
     """Cell average of `factor` consecutive velocity cells: f [n, nx, nv] ->
     [n, nx, nv / factor] (conserves sum f dv)."""
@@ -205,16 +267,33 @@ def training_pairs(inst, n_steps, every, span=1, fine_precision="fp64", log=None
     return torch.cat(xs), torch.cat(ts)
 
 
-def train(x, t, steps, batch=4096, lr=1e-3, seed=0, net=None, log=None):
+def train(x, t, steps, batch=4096, lr=1e-3, seed=0, net=None, log=None, bg=None, conserve=False):
     """Adam on minibatches of rows drawn without replacement per epoch.
-    Scales: in_scale = 1 / max |x|, out_scale = rms(t) (targets of order 1)."""
-    in_scale = 1.0 / float(x.abs().max())
-    out_scale = float(t.pow(2).mean().sqrt())
+    With a background bg the network works on d / s (d = x - bg, s = rms_v d
+    per row) and predicts t / s.  Scales: in_scale = 1 / max |d / s|,
+    out_scale = rms(t / s), so that inputs and targets are of order 1."""
+    if conserve:
+        # targets projected like the corrections the kernel will apply
+        P = torch.as_tensor(conservation_projector(coarse_v(), np.asarray(background(), np.float64)),
+                            dtype=torch.float32, device=t.device)
+        t = (t @ P.T).contiguous()
+    if bg is not None:
+        d = x - torch.as_tensor(bg, device=x.device)[None, :]
+        srow = (d.pow(2).mean(-1, keepdim=True) + 1e-30).sqrt()
+        in_scale = 1.0 / float((d / srow).abs().max())
+        out_scale = float((t / srow).pow(2).mean().sqrt())
+        del d, srow
+    else:
+        in_scale = 1.0 / float(x.abs().max())
+        out_scale = float(t.pow(2).mean().sqrt())
     tn = (t / out_scale).contiguous()
     if net is None:
-        net = Closure(in_scale=in_scale, out_scale=out_scale, max_rows=batch, seed=seed)
+        net = Closure(in_scale=in_scale, out_scale=out_scale, max_rows=batch, seed=seed, bg=bg)
     else:
         net.set_scales(in_scale, out_scale)
+        net.set_background(bg)
+    if conserve:
+        net.set_conservation(coarse_v(), np.asarray(background(), np.float64))
     gen = torch.Generator(device=x.device)
     gen.manual_seed(seed)
     loss = torch.zeros(1, dtype=torch.float32, device=x.device)
@@ -238,13 +317,25 @@ def train(x, t, steps, batch=4096, lr=1e-3, seed=0, net=None, log=None):
 
 def rollout(inst, nv, precision, n_steps, net=None):
     """<E^2>(t) [n_steps, n] of plain runs on an nv grid, or of FP32 coarse
-    runs with the closure applied after every step (net given)."""
+    runs with the closure applied after every step (net given).  Coarse runs
+    start from the fine initial state averaged onto their grid,
+    (1 + amp cos kx) background(), the projection the closure is trained on."""
     sc = _scan(inst, nv, precision)
+    if nv == NV:
+        from .scan import instance_grids
+
+        c = sc.classes[0]
+        x = np.stack([instance_grids(NX, k, NV, VMAX)[0] for k in inst["k"]])
+        dens = 1.0 + inst["amp"][:, None] * np.cos(inst["k"][:, None] * x)
+        f0 = dens[:, :, None] * background().astype(np.float64)[None, None, :]
+        c.f.copy_(torch.as_tensor(f0, device=c.f.device).to(c.f.dtype))
     if net is not None:
         _lib.check(_lib.library().vpfp_batch_set_closure(sc.classes[0].handle, net.handle))
+    n0 = sc.density()
     e2 = sc.run(n_steps)
     if net is not None:
         _lib.check(_lib.library().vpfp_batch_set_closure(sc.classes[0].handle, None))
+    rollout.density_drift = np.abs(sc.density() / n0 - 1.0)
     return e2
 
 

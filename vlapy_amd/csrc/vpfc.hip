@@ -124,6 +124,11 @@ struct FwdArgs {
   const float* f;       // [rows][256]
   int rows;
   float in_scale, out_scale;
+  const float* bg;      // [256] background profile, or null: the network then sees
+                        // (f - bg) / s with s the row's rms of f - bg, and its output and
+                        // target are in units of s
+  const float* cons;    // [256 * 4 + 9] conservation projection (v, w, bg, unused, G^-1)
+                        // applied to the correction, or null
   const float* p;       // FP32 parameters (W1, b1, b2, b3, W4, b4 are read here)
   const bf16* w2;       // [64][320] BF16 copies
   const bf16* w3;
@@ -141,6 +146,16 @@ struct FwdArgs {
 };
 
 constexpr int kFwdLds = 2 * kRowsP * kP * 2 + kRowsP * 4 + 64 + 2 * kK * 4;
+
+// sum over the 256 threads of a workgroup (all threads get it); red: 4 floats
+__device__ __forceinline__ float block_sum(float v, float* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
 
 __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -161,7 +176,13 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
   for (int row = blockIdx.x; row < a.rows; row += gridDim.x) {
     const size_t ro = (size_t)row * kNV;
     const float fv = a.f[ro + t];
-    const float x = bfr(a.in_scale * fv);
+    // row scale: rms of the deviation from the background (1 without one)
+    float d = fv, srow = 1.f;
+    if (a.bg) {
+      d = fv - a.bg[t];
+      srow = sqrtf(block_sum(d * d, red) * (1.f / kNV) + 1e-30f);
+    }
+    const float x = bfr((a.in_scale / srow) * d);
     xs[t + 2] = x;
     if (a.xin) a.xin[ro + t] = (bf16)x;
     __syncthreads();
@@ -232,17 +253,25 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
         for (int j = 0; j < 8; ++j) y = fmaf(w4r[k * kCH + c0 + j], (float)v[j], y);
       }
     }
+    if (a.cons) {
+      // remove bg (c0 + c1 v + c2 v^2): trapz moments 1, v, v^2 of y vanish
+      const float vv = a.cons[t], w = a.cons[kNV + t], b = a.cons[2 * kNV + t];
+      const float m0 = block_sum(w * y, red);
+      const float m1 = block_sum(w * vv * y, red);
+      const float m2 = block_sum(w * vv * vv * y, red);
+      const float* gi = a.cons + 4 * kNV;
+      const float c0 = gi[0] * m0 + gi[1] * m1 + gi[2] * m2;
+      const float c1 = gi[3] * m0 + gi[4] * m1 + gi[5] * m2;
+      const float c2 = gi[6] * m0 + gi[7] * m1 + gi[8] * m2;
+      y -= b * (c0 + vv * (c1 + vv * c2));
+    }
     if (a.y_out) a.y_out[ro + t] = y;
-    if (a.f_out) a.f_out[ro + t] = fv + a.out_scale * y;
+    if (a.f_out) a.f_out[ro + t] = fv + (a.out_scale * srow) * y;
     if (a.dy) {
-      const float d = y - a.target[ro + t];
-      a.dy[ro + t] = a.gscale * d;
-      float s = d * d;
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-      if ((t & 63) == 0) red[t >> 6] = s;
-      __syncthreads();
-      if (t == 0) a.loss_rows[row] = (red[0] + red[1]) + (red[2] + red[3]);
+      const float e = y - a.target[ro + t] / srow;
+      a.dy[ro + t] = a.gscale * e;
+      const float s = block_sum(e * e, red);
+      if (t == 0) a.loss_rows[row] = s;
     }
     __syncthreads();
   }
@@ -546,6 +575,10 @@ __global__ __launch_bounds__(256) void k_adam(float* p, float* mo, float* ve, co
 struct vpfc_net_s {
   int max_rows = 0;
   float in_scale = 1.f, out_scale = 1.f;
+  float* bg = nullptr;       // [256] background profile (device)
+  bool has_bg = false;
+  float* cons = nullptr;     // [256 * 4 + 9] conservation projection (device)
+  bool has_cons = false;
   int adam_t = 0;
   float* p = nullptr;
   float* mo = nullptr;
@@ -606,6 +639,8 @@ FwdArgs fwd_args(vpfc_net_t n, const float* x, int rows) {
   a.p = n->p;
   a.w2 = n->w2;
   a.w3 = n->w3;
+  a.bg = n->has_bg ? n->bg : nullptr;
+  a.cons = n->has_cons ? n->cons : nullptr;
   return a;
 }
 
@@ -619,6 +654,7 @@ int run_fwd(const FwdArgs& a, hipStream_t s) {
 int gradients(vpfc_net_t n, const float* x, const float* target, int rows, float* grad, float* loss,
               hipStream_t s) {
   FwdArgs a = fwd_args(n, x, rows);
+  a.cons = nullptr;  // the backward kernels differentiate the unprojected output
   a.xin = n->xin;
   a.a1 = n->a1;
   a.a2 = n->a2;
@@ -680,6 +716,8 @@ int vpfc_create(int max_rows, vpfc_net_t* out) {
   if (!rc) rc = nalloc(n, &n->dy, R * kNV * sizeof(float));
   if (!rc) rc = nalloc(n, &n->loss_rows, R * sizeof(float));
   if (!rc) rc = nalloc(n, &n->loss, sizeof(float));
+  if (!rc) rc = nalloc(n, &n->bg, sizeof(float) * kNV);
+  if (!rc) rc = nalloc(n, &n->cons, sizeof(float) * (4 * kNV + 9));
   if (!rc) {
     std::vector<float> z(kParams, 0.f);
     hipError_t e = hipMemcpy(n->p, z.data(), sizeof(float) * kParams, hipMemcpyHostToDevice);
@@ -738,6 +776,53 @@ int vpfc_set_scales(vpfc_net_t n, float in_scale, float out_scale) {
   if (!std::isfinite(in_scale) || !std::isfinite(out_scale)) return fail(VPFP_EBADARG, "non-finite scale");
   n->in_scale = in_scale;
   n->out_scale = out_scale;
+  return VPFP_OK;
+}
+
+int vpfc_set_background(vpfc_net_t n, const float* host_bg) {
+  if (int rc = check_net(n)) return rc;
+  FC_HIP(hipDeviceSynchronize());
+  if (host_bg) FC_HIP(hipMemcpy(n->bg, host_bg, sizeof(float) * kNV, hipMemcpyHostToDevice));
+  n->has_bg = host_bg != nullptr;
+  return VPFP_OK;
+}
+
+int vpfc_set_conservation(vpfc_net_t n, const double* v, const double* bg) {
+  if (int rc = check_net(n)) return rc;
+  FC_HIP(hipDeviceSynchronize());
+  if (!v || !bg) {
+    n->has_cons = false;
+    return VPFP_OK;
+  }
+  // G[l][k] = sum_v w v^(l+k) bg(v) with trapz weights (1, ends 1/2)
+  double G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  std::vector<float> h(4 * kNV + 9, 0.f);
+  for (int j = 0; j < kNV; ++j) {
+    const double w = (j == 0 || j == kNV - 1) ? 0.5 : 1.0;
+    double pw[5] = {1, v[j], v[j] * v[j], v[j] * v[j] * v[j], v[j] * v[j] * v[j] * v[j]};
+    for (int l = 0; l < 3; ++l)
+      for (int k = 0; k < 3; ++k) G[l][k] += w * pw[l + k] * bg[j];
+    h[j] = (float)v[j];
+    h[kNV + j] = (float)w;
+    h[2 * kNV + j] = (float)bg[j];
+  }
+  // inverse of the symmetric 3x3 by cofactors
+  const double c00 = G[1][1] * G[2][2] - G[1][2] * G[2][1], c01 = G[1][2] * G[2][0] - G[1][0] * G[2][2],
+               c02 = G[1][0] * G[2][1] - G[1][1] * G[2][0];
+  const double det = G[0][0] * c00 + G[0][1] * c01 + G[0][2] * c02;
+  if (!(std::fabs(det) > 0.0)) return fail(VPFP_EBADARG, "singular moment matrix of the background");
+  const double inv[9] = {c00 / det,
+                         (G[0][2] * G[2][1] - G[0][1] * G[2][2]) / det,
+                         (G[0][1] * G[1][2] - G[0][2] * G[1][1]) / det,
+                         c01 / det,
+                         (G[0][0] * G[2][2] - G[0][2] * G[2][0]) / det,
+                         (G[0][2] * G[1][0] - G[0][0] * G[1][2]) / det,
+                         c02 / det,
+                         (G[0][1] * G[2][0] - G[0][0] * G[2][1]) / det,
+                         (G[0][0] * G[1][1] - G[0][1] * G[1][0]) / det};
+  for (int i = 0; i < 9; ++i) h[4 * kNV + i] = (float)inv[i];
+  FC_HIP(hipMemcpy(n->cons, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice));
+  n->has_cons = true;
   return VPFP_OK;
 }
 

@@ -480,7 +480,10 @@ template <class T, int M, int Q, int C>
 struct DriftCfg {
   static constexpr int TPT = M / Q;
   static constexpr int wg = TPT * C;
-  static constexpr int stride = Padded<M>::size;
+  // odd stride (in complex elements): the C column buffers of a tile start
+  // on different LDS banks (an even multiple of 64 banks made the columns a
+  // wave touches collide)
+  static constexpr int stride = Padded<M>::size | 1;
   static constexpr int tw_bytes = (TwSize<2 * M>::total * (int)sizeof(cx<T>) + 15) & ~15;
   static constexpr int own = tw_bytes + C * stride * (int)sizeof(cx<T>);
   // fused field tail (small grids): M-point Poisson transform on the same workgroup

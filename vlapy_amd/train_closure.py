@@ -36,10 +36,13 @@ def judge(net, inst, steps):
         e2 = CL.rollout(inst, CL.NV, "fp32", steps, n)
         torch.cuda.synchronize()
         sec = time.perf_counter() - t0
+        drift = CL.rollout.density_drift
         h = CL.history_errors(ref, e2)
         d = CL.damping_errors(inst, ref, e2, t)
         ok = ~np.isnan(d)
-        out[name] = {"seconds": sec, "history_err_median": float(np.median(h)), "history_err_mean": float(np.mean(h)),
+        out[name] = {"seconds": sec, "density_drift_max": float(np.max(drift)),
+                     "finite": bool(np.all(np.isfinite(e2))),
+                     "history_err_median": float(np.median(h)), "history_err_mean": float(np.mean(h)),
                      "damping_fitted": int(ok.sum()), "damping_err_median": float(np.median(d[ok])),
                      "damping_within_1pct": float(np.mean(d[ok] <= 0.01))}
     return out
@@ -57,6 +60,8 @@ def main():
     ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--eval-instances", type=int, default=128)
     ap.add_argument("--eval-seed", type=int, default=12)
+    ap.add_argument("--relative", type=int, default=0, help="1: input relative to the background, per-row scale")
+    ap.add_argument("--conserve", type=int, default=1, help="1: corrections add no density, momentum, energy")
     ap.add_argument("--out", required=True)
     a = ap.parse_args()
     t0 = time.perf_counter()
@@ -66,6 +71,7 @@ def main():
     print("pairs %d rows, %.1f s" % (x.shape[0], time.perf_counter() - t0), flush=True)
     t0 = time.perf_counter()
     net, hist = CL.train(x, t, a.train_steps, batch=a.batch, lr=a.lr, seed=a.seed,
+                         bg=CL.background() if a.relative else None, conserve=a.conserve,
                          log=lambda it, l: print("  step %d loss %.4f" % (it, l), flush=True))
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
