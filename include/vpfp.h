@@ -158,6 +158,8 @@ int vpfp_batch_destroy(vpfp_batch_t batch);
  *               count (4 when the grid has few rows, else 8)
  *   drift_tile  x-advection columns per workgroup: 4, 8, 16 or 32 (0 = keep)
  *   fuse_field  1: the last drift workgroup solves the field (nx <= 256);
+ *               2: every row workgroup forms the field at its own rows from
+ *               the drift's partial densities (direct DFT, nx <= 128);
  *               0 or -1 (default): separate field launch.  Batches always
  *               use the separate launch and report 0.
  *   drift_order 1: consecutive column tiles on the workgroups of one XCD

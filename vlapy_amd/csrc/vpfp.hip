@@ -52,7 +52,7 @@ struct GpuCtx {
 
 // ------------------------------------------------------------------ kernels
 template <class T, int M, int Q>
-__global__ __launch_bounds__((vp::RowCfg<T, M, Q>::wg)) void k_row(vp::RowArgs<T> a) {
+__global__ __launch_bounds__((vp::RowCfg<T, M, Q>::wg), (vp::RowCfg<T, M, Q>::min_waves)) void k_row(vp::RowArgs<T> a) {
   extern __shared__ __align__(16) unsigned char lds[];
   VPFP_CTX;
   vp::row_body<T, M, Q>(c, lds, a);
@@ -606,23 +606,31 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
   ra.flags = vp::RK_KICK_A;
   if (n > 0) PROF(KK_ROW, row_dispatch<T>(ra, s, p->row_q));
   const double* e_cur = e;
-  const bool fuse = p->fuse >= 0 ? p->fuse == 1 : default_fuse(p->nx);
+  // field: 0 own launch, 1 tail of the drift, 2 prologue of the row kernel
+  const int mode = p->fuse >= 0 ? p->fuse : (default_fuse(p->nx) ? 1 : 0);
+  if (mode == 2 && p->nx > vp::kProNx)
+    return fail(VPFP_EUNSUPPORTED, "field in the row kernel needs nx <= " + std::to_string(vp::kProNx));
   for (int i = 0; i < n; ++i) {
     double* ei = e_hist ? e_hist + (size_t)i * p->nx : p->ering + (size_t)(i & 1) * p->nx;
-    auto fa = field_args(p, p->dens, p->ntiles, p->dv, driver ? driver + (size_t)i * p->nx : nullptr, ei, nullptr,
-                         series ? series + 2 * (size_t)i : nullptr, true);
-    if (fuse) {
+    const double* drv = driver ? driver + (size_t)i * p->nx : nullptr;
+    double* ser = series ? series + 2 * (size_t)i : nullptr;
+    auto fa = field_args(p, p->dens, p->ntiles, p->dv, drv, ei, nullptr, ser, true);
+    if (mode == 1) {
       auto da = drift_args<T>(p, f, f, p->dt, p->dens);
       da.fuse_field = 1;
       da.fld = fa;
       PROF(KK_DRIFT, drift_dispatch<T>(da, p->C, p->ntiles, s));
     } else {
       PROF(KK_DRIFT, (do_vdfdx<T>(p, f, f, p->dt, p->dens, s)));
-      PROF(KK_FIELD, field_dispatch(fa, true, s));
+      if (mode == 0) PROF(KK_FIELD, field_dispatch(fa, true, s));
     }
     e_cur = ei;
     auto rb = row_args<T>(p, f, f);
     rb.e = e_cur;
+    if (mode == 2) {
+      rb.field_pro = 1;
+      rb.fld = field_args(p, p->dens, p->ntiles, p->dv, drv, ei, nullptr, ser, false);
+    }
     rb.dt_a = half;
     rb.dt_b = half;
     rb.moments = moments ? moments + (size_t)i * vp::kMoments * p->nx : nullptr;
@@ -885,8 +893,9 @@ int autotune_plan(vpfp_plan_t p, int n_steps, hipStream_t s) {
   for (int q : row_q_candidates(p->nv))
     for (int C : {4, 8, 16, 32}) {
       if (C > p->nv) continue;
-      for (int fuse : {0, 1}) {
-        if (fuse && p->nx > 256) continue;
+      for (int fuse : {0, 1, 2}) {
+        if (fuse == 1 && p->nx > 256) continue;
+        if (fuse == 2 && p->nx > vp::kProNx) continue;
         for (int xo : {0, 1})
           for (int dt : {1, 2, 4}) {
             // more tiles per workgroup only where it changes the grid
@@ -943,7 +952,7 @@ int check_tuning(const vpfp_tuning* t, int nv) {
     return fail(VPFP_EBADARG, "drift_tile must be 0, 4, 8, 16 or 32");
   if (t->drift_order != 0 && t->drift_order != 1) return fail(VPFP_EBADARG, "drift_order must be 0 or 1");
   if (t->drift_tile > nv) return fail(VPFP_EBADARG, "drift_tile exceeds nv");
-  if (t->fuse_field < -1 || t->fuse_field > 1) return fail(VPFP_EBADARG, "fuse_field must be -1, 0 or 1");
+  if (t->fuse_field < -1 || t->fuse_field > 2) return fail(VPFP_EBADARG, "fuse_field must be -1, 0, 1 or 2");
   return VPFP_OK;
 }
 

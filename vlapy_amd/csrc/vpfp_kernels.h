@@ -103,6 +103,137 @@ VP_HD void coll_moments(Ctx& c, const double* d, double v0, double dvv, int N, d
 }
 
 // ---------------------------------------------------------------------------
+// field kernel: n(x) = scale * sum_tiles part[tile][x];  E = Re ifft(i/kx fft(1 - n)) + driver
+// ---------------------------------------------------------------------------
+struct FieldArgs {
+  const double* part;
+  int ntiles;
+  int nx;
+  double scale;
+  double* rho_g;          // [nx] global scratch when the grid has several workgroups
+  unsigned int* counter;  // arrival counter (zero between launches)
+  const double* okx;      // [nx]
+  const cx<double>* tw;   // pow2 kernel: twiddle tables [lo | hi]; DFT kernel: exp(-2 pi i n / nx), n < nx
+  const double* driver;   // [nx] or null
+  double* e_out;          // [nx]
+  double* e_out2;         // [nx] or null
+  double* series;         // [2] or null: mean(E^2), mean(driver^2)
+  int series_stride;      // batched scan: doubles between consecutive instances' series entries
+};
+
+// The view of instance inst of a batched field solve: every array is
+// instance-major (part [inst][ntiles][nx], okx / driver / E [inst][nx],
+// series [inst * series_stride], one 64-byte counter slot per instance).
+// Instance 0 is the single-grid case itself.
+VP_HD FieldArgs field_instance(const FieldArgs& a, int inst) {
+  FieldArgs r = a;
+  if (inst == 0) return r;
+  const size_t xo = (size_t)inst * a.nx;
+  r.part += (size_t)inst * a.ntiles * a.nx;
+  if (r.rho_g) r.rho_g += xo;
+  if (r.counter) r.counter += inst * 16;
+  r.okx += xo;
+  if (r.driver) r.driver += xo;
+  r.e_out += xo;
+  if (r.e_out2) r.e_out2 += xo;
+  if (r.series) r.series += (size_t)inst * a.series_stride;
+  return r;
+}
+
+// rho[x] = 1 - scale * sum_tiles part[tile][x] for x in [x0, x0 + XS)
+// (all FT threads; XS divides FT or FT divides XS)
+template <int FT, bool COH = false, class Ctx>
+VP_HD void field_density(Ctx& c, const FieldArgs& a, int x0, int XS, double* rho, double* tmp) {
+  const int nx = a.nx;
+  const int G = XS >= FT ? 1 : FT / XS;
+  for (int l0 = c.tid; l0 < XS * G; l0 += FT) {
+    const int xl = l0 % XS, g = l0 / XS;
+    const double* p = a.part + x0 + xl;
+    auto ld = [&](int tl) { return COH ? c.load_coherent(p + (size_t)tl * nx) : p[(size_t)tl * nx]; };
+    // eight independent partial sums keep eight loads in flight per thread
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    int tile = g;
+    for (; tile + 7 * G < a.ntiles; tile += 8 * G) {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] += ld(tile + u * G);
+    }
+    for (int u = 0; tile < a.ntiles; tile += G, ++u) acc[u & 7] += ld(tile);
+    tmp[l0] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+  }
+  c.sync();
+  for (int xl = c.tid; xl < XS; xl += FT) {
+    double s = 0.0;
+    for (int g = 0; g < G; ++g) s += tmp[g * XS + xl];
+    rho[xl] = 1.0 - a.scale * s;
+  }
+  c.sync();
+}
+
+// Field at a workgroup's own rows from the drift's partial density rows, by
+// the direct DFT of field_dft_body (small single grids, nx <= kProNx):
+//   E(x) = Re sum_m F_m exp(2 pi i m x / nx) / nx + driver(x),
+//   F_m = i g_m sum_j rho_j exp(-2 pi i m j / nx), g_m = (okx_m - okx_{nx-m}) / 2.
+// Writes E of rows [row0, row0 + nrows) to e_out (and e_out2); the workgroup
+// holding row 0 also forms all E for the series.  All FT threads call it.
+// lds: rho [kProNx] | F [kProNx] complex | tmp [max(2 kProNx, FT)] | erow [nrows]
+constexpr int kProNx = 128;
+template <int FT>
+struct ProCfg {
+  static constexpr int tmp_n = 2 * kProNx > FT ? 2 * kProNx : FT;
+  static constexpr int bytes = kProNx * 8 + kProNx * 16 + tmp_n * 8 + 64 * 8;
+};
+
+template <int FT, class Ctx>
+VP_HD void field_rows(Ctx& c, const FieldArgs& a, unsigned char* lds, int row0, int nrows) {
+  const int nx = a.nx;
+  double* rho = reinterpret_cast<double*>(lds);
+  cx<double>* F = reinterpret_cast<cx<double>*>(lds + kProNx * 8);
+  double* tmp = reinterpret_cast<double*>(lds + kProNx * 24);
+  double* erow = tmp + ProCfg<FT>::tmp_n;
+  field_density<FT>(c, a, 0, nx, rho, tmp);
+  for (int m = c.tid; m < nx; m += FT) {
+    cx<double> s = mk<double>(0.0, 0.0);
+    for (int j = 0; j < nx; ++j) {
+      const cx<double> w = a.tw[(int)(((long long)m * j) % nx)];
+      s.re += rho[j] * w.re;
+      s.im += rho[j] * w.im;
+    }
+    const double g = 0.5 * (a.okx[m] - a.okx[(nx - m) % nx]);
+    F[m] = mk<double>(-g * s.im, g * s.re);
+  }
+  c.sync();
+  auto ex = [&](int x) {
+    double s = 0.0;
+    for (int m = 0; m < nx; ++m) {
+      const cx<double> w = a.tw[(int)(((long long)m * x) % nx)];
+      s += F[m].re * w.re + F[m].im * w.im;
+    }
+    return s / nx + (a.driver ? a.driver[x] : 0.0);
+  };
+  for (int r = c.tid; r < nrows; r += FT) {
+    const double e = ex(row0 + r);
+    erow[r] = e;
+    a.e_out[row0 + r] = e;
+    if (a.e_out2) a.e_out2[row0 + r] = e;
+  }
+  if (row0 == 0 && a.series) {
+    double acc[2] = {0.0, 0.0};
+    for (int x = c.tid; x < nx; x += FT) {
+      const double e = ex(x);
+      const double dr = a.driver ? a.driver[x] : 0.0;
+      acc[0] += e * e;
+      acc[1] += dr * dr;
+    }
+    row_sum<FT, 2>(c, acc, tmp);
+    if (c.tid == 0) {
+      a.series[0] = acc[0] / nx;
+      a.series[1] = acc[1] / nx;
+    }
+  }
+  c.sync();
+}
+
+// ---------------------------------------------------------------------------
 // row kernel
 // ---------------------------------------------------------------------------
 template <class T>
@@ -122,6 +253,10 @@ struct RowArgs {
   int flags;
   const double* nu_dt_inst;  // batched scan: nu*dt per instance (rows are instance-major), or null
   int rows_per_inst;         // x rows per instance in a batch
+  // field in the prologue (field_rows): E of the workgroup's rows from the
+  // drift's partial densities instead of e (single grid, nx <= kProNx)
+  int field_pro;
+  FieldArgs fld;
 };
 
 template <class T, int M, int Q>
@@ -135,7 +270,15 @@ struct RowCfg {
   static constexpr int wg = TPT >= 64 ? TPT : 64;
   static constexpr int rows = wg / TPT;
   static constexpr int sum_bytes = (TPT > 64 ? TPT / 64 : 1) * kMoments * 8;
-  static constexpr int lds = tw_bytes + rows * region + sum_bytes;
+  // the field prologue (field_rows) runs before the twiddles and row buffers
+  // are staged and overlays them from offset 0
+  static constexpr int pro_bytes = ProCfg<wg>::bytes;
+  static constexpr int main_bytes = tw_bytes + rows * region + sum_bytes;
+  static constexpr int lds = main_bytes > pro_bytes ? main_bytes : pro_bytes;
+  // minimum waves per SIMD asked of the register allocator: 3 (168 VGPRs,
+  // small spills) measured faster than 2 for FP32 rows with M <= 1024 (the
+  // scan's classes) and slower for FP64 and for M = 4096
+  static constexpr int min_waves = (Q == 8 && sizeof(T) == 4 && M <= 1024) ? 3 : 1;
 };
 
 template <class T, int M, int Q, class Ctx>
@@ -158,7 +301,16 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
   cx<T> v[Q];
 #pragma unroll
   for (int i = 0; i < Q; ++i) v[i] = src[t + i * TPT];
-  const double e = (a.flags & (RK_KICK_A | RK_KICK_B)) ? a.e[rowc] : 0.0;
+  double e = 0.0;
+  if (a.field_pro) {
+    const int row0 = c.bid * Cfg::rows;
+    const int nr = a.nx - row0 < Cfg::rows ? a.nx - row0 : Cfg::rows;
+    field_rows<Cfg::wg>(c, a.fld, lds, row0, nr);
+    e = reinterpret_cast<const double*>(lds + kProNx * 24)[ProCfg<Cfg::wg>::tmp_n + (valid ? rl : 0)];
+    c.sync();  // every thread has its E before the twiddles overwrite the prologue's LDS
+  } else if (a.flags & (RK_KICK_A | RK_KICK_B)) {
+    e = a.e[rowc];
+  }
   tw_stage<T, 2 * M>(c, twl, a.tw);
   c.sync();
   const Tw<T, TwSize<2 * M>::full> tw{twl};
@@ -239,73 +391,6 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
 #pragma unroll
     for (int i = 0; i < Q; ++i) dst[t + i * TPT] = v[i];
   }
-}
-
-// ---------------------------------------------------------------------------
-// field kernel: n(x) = scale * sum_tiles part[tile][x];  E = Re ifft(i/kx fft(1 - n)) + driver
-// ---------------------------------------------------------------------------
-struct FieldArgs {
-  const double* part;
-  int ntiles;
-  int nx;
-  double scale;
-  double* rho_g;          // [nx] global scratch when the grid has several workgroups
-  unsigned int* counter;  // arrival counter (zero between launches)
-  const double* okx;      // [nx]
-  const cx<double>* tw;   // pow2 kernel: twiddle tables [lo | hi]; DFT kernel: exp(-2 pi i n / nx), n < nx
-  const double* driver;   // [nx] or null
-  double* e_out;          // [nx]
-  double* e_out2;         // [nx] or null
-  double* series;         // [2] or null: mean(E^2), mean(driver^2)
-  int series_stride;      // batched scan: doubles between consecutive instances' series entries
-};
-
-// The view of instance inst of a batched field solve: every array is
-// instance-major (part [inst][ntiles][nx], okx / driver / E [inst][nx],
-// series [inst * series_stride], one 64-byte counter slot per instance).
-// Instance 0 is the single-grid case itself.
-VP_HD FieldArgs field_instance(const FieldArgs& a, int inst) {
-  FieldArgs r = a;
-  if (inst == 0) return r;
-  const size_t xo = (size_t)inst * a.nx;
-  r.part += (size_t)inst * a.ntiles * a.nx;
-  if (r.rho_g) r.rho_g += xo;
-  if (r.counter) r.counter += inst * 16;
-  r.okx += xo;
-  if (r.driver) r.driver += xo;
-  r.e_out += xo;
-  if (r.e_out2) r.e_out2 += xo;
-  if (r.series) r.series += (size_t)inst * a.series_stride;
-  return r;
-}
-
-// rho[x] = 1 - scale * sum_tiles part[tile][x] for x in [x0, x0 + XS)
-// (all FT threads; XS divides FT or FT divides XS)
-template <int FT, bool COH = false, class Ctx>
-VP_HD void field_density(Ctx& c, const FieldArgs& a, int x0, int XS, double* rho, double* tmp) {
-  const int nx = a.nx;
-  const int G = XS >= FT ? 1 : FT / XS;
-  for (int l0 = c.tid; l0 < XS * G; l0 += FT) {
-    const int xl = l0 % XS, g = l0 / XS;
-    const double* p = a.part + x0 + xl;
-    auto ld = [&](int tl) { return COH ? c.load_coherent(p + (size_t)tl * nx) : p[(size_t)tl * nx]; };
-    // eight independent partial sums keep eight loads in flight per thread
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    int tile = g;
-    for (; tile + 7 * G < a.ntiles; tile += 8 * G) {
-#pragma unroll
-      for (int u = 0; u < 8; ++u) acc[u] += ld(tile + u * G);
-    }
-    for (int u = 0; tile < a.ntiles; tile += G, ++u) acc[u & 7] += ld(tile);
-    tmp[l0] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-  }
-  c.sync();
-  for (int xl = c.tid; xl < XS; xl += FT) {
-    double s = 0.0;
-    for (int g = 0; g < G; ++g) s += tmp[g * XS + xl];
-    rho[xl] = 1.0 - a.scale * s;
-  }
-  c.sync();
 }
 
 // Several workgroups: each reduces an x slice into rho_g; the last one to
