@@ -212,6 +212,7 @@ def roofline(run, cells, ms_per_step, traffic_file=None):
 # --------------------------------------------------------------------------- scan
 
 SCAN_N, SCAN_SEED, SCAN_STEPS, SCAN_DT = 4096, 20261202, 2000, 0.1
+SCAN_GRAPH_STEPS = 100
 
 
 def scan_window(p, i, e2, t):
@@ -246,7 +247,21 @@ def scan_run(precision, steps):
     e2 = sc.run(steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return p, sc.cells, e2, wall, tuning
+    # the same steps replayed from one HIP graph of SCAN_GRAPH_STEPS steps
+    # (from the initial state again; no <E^2> record)
+    graph_wall = None
+    if steps % SCAN_GRAPH_STEPS == 0:
+        del sc
+        sc = Scan(p, dt=SCAN_DT, op="dg", precision=precision)
+        for c, (k, t) in zip(sc.classes, tuning.items()):
+            c.set_tuning(t)
+        sc.capture(SCAN_GRAPH_STEPS)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sc.replay(steps // SCAN_GRAPH_STEPS)
+        torch.cuda.synchronize()
+        graph_wall = time.perf_counter() - t0
+    return p, sc.cells, e2, wall, tuning, graph_wall
 
 
 def scan_cpu(p, steps_sample=10, n_sample=12):
@@ -296,8 +311,8 @@ def scan_section(steps, with_cpu):
 
     from vlapy_amd.diagnostics.low_level_helpers import get_standing_wave_rate_and_frequency as rate
 
-    p, cells, e2_32, wall32, tun32 = scan_run("fp32", steps)
-    _, _, e2_64, wall64, _ = scan_run("fp64", steps)
+    p, cells, e2_32, wall32, tun32, gwall32 = scan_run("fp32", steps)
+    _, _, e2_64, wall64, _, _ = scan_run("fp64", steps)
     t = SCAN_DT * (np.arange(steps) + 1)
     errs = []
     for i in range(SCAN_N):
@@ -312,6 +327,7 @@ def scan_section(steps, with_cpu):
     out = {
         "workload": "4096-instance ragged scan, seed 20261202, Dougherty, dt=0.1, %d steps" % steps,
         "instances_per_s": SCAN_N / wall32,
+        "graph_instances_per_s": SCAN_N / gwall32 if gwall32 else None,
         "cell_updates_per_s": cells * steps / wall32,
         "seconds": wall32,
         "fp64_instances_per_s": SCAN_N / wall64,

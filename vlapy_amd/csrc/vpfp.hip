@@ -378,7 +378,7 @@ std::vector<cx<T>> twiddles(int N) {
 // N <= vp::kTwFull, else [lo | hi]: lo[j] = W_N^j (j < 64), hi[h] = W_N^{64 h}
 template <class T>
 std::vector<cx<T>> split_twiddles(int N) {
-  if (vp::tw_full(N)) return twiddles<T>(N);
+  if (vp::tw_full(N, (int)sizeof(cx<T>))) return twiddles<T>(N);
   const int nhi = N / 64 > 1 ? N / 64 : 1;
   std::vector<cx<T>> w(64 + nhi);
   const long double two_pi = 6.283185307179586476925286766559L;

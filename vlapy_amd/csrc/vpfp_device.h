@@ -180,15 +180,20 @@ VP_HD void dft(cx<T>* x) {
 // tables, W_N^i = hi[i >> 6] * lo[i & 63], hi[h] = W_N^{64 h}, lo[j] = W_N^j,
 // global layout [lo: 64 entries | hi: max(1, N/64) entries].
 // ---------------------------------------------------------------------------
-// For N <= kTwFull the whole table W_N^i, i < N, is staged instead.
+// For N <= kTwFull the whole table W_N^i, i < N, is staged instead: one LDS
+// read per twiddle.  (A full FP32 table for N = 2048, 16 KiB, measured no
+// faster than the two-level one at nv = 2048.)
 constexpr int kTwFull = 1024;
-VP_HD constexpr bool tw_full(int N) { return N <= kTwFull; }
-VP_HD constexpr int tw_entries(int N) { return tw_full(N) ? N : 64 + (N / 64 > 1 ? N / 64 : 1); }
+VP_HD constexpr bool tw_full(int N, int bytes) { return bytes > 0 && N <= kTwFull; }
+VP_HD constexpr int tw_entries(int N, int bytes) {
+  return tw_full(N, bytes) ? N : 64 + (N / 64 > 1 ? N / 64 : 1);
+}
 
-template <int N>
+// B: bytes of one complex entry (sizeof(cx<T>))
+template <int N, int B>
 struct TwSize {
-  static constexpr bool full = tw_full(N);
-  static constexpr int total = tw_entries(N);
+  static constexpr bool full = tw_full(N, B);
+  static constexpr int total = tw_entries(N, B);
 };
 
 template <class T, bool FULL = false>
@@ -208,7 +213,7 @@ struct Tw {
 // all threads copy the table to LDS (no barrier here)
 template <class T, int N, class Ctx>
 VP_HD void tw_stage(Ctx& c, cx<T>* dst, const cx<T>* src) {
-  for (int i = c.tid; i < TwSize<N>::total; i += c.nthr) dst[i] = src[i];
+  for (int i = c.tid; i < TwSize<N, (int)sizeof(cx<T>)>::total; i += c.nthr) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------

@@ -263,7 +263,7 @@ template <class T, int M, int Q>
 struct RowCfg {
   static constexpr int TPT = M / Q;
   static constexpr int L = 2 * Q;
-  static constexpr int tw_bytes = (TwSize<2 * M>::total * (int)sizeof(cx<T>) + 15) & ~15;
+  static constexpr int tw_bytes = (TwSize<2 * M, (int)sizeof(cx<T>)>::total * (int)sizeof(cx<T>) + 15) & ~15;
   static constexpr int fft_bytes = Padded<M>::size * (int)sizeof(cx<T>);
   static constexpr int red_bytes = 8 * TPT * 8;
   static constexpr int region = ((fft_bytes > red_bytes ? fft_bytes : red_bytes) + 15) & ~15;
@@ -313,7 +313,7 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
   }
   tw_stage<T, 2 * M>(c, twl, a.tw);
   c.sync();
-  const Tw<T, TwSize<2 * M>::full> tw{twl};
+  const Tw<T, TwSize<2 * M, (int)sizeof(cx<T>)>::full> tw{twl};
 
   if (a.flags & RK_KICK_A)
     spectral_apply<T, M, TPT>(c, v, buf, tw, t, PhaseMul<T>((a.dk * a.dt_a) * e, M, t, TPT));
@@ -434,7 +434,7 @@ struct FieldCfg {
   static constexpr int TPT = M / Q;
   static constexpr int nx = 2 * M;
   static constexpr int tmp_n = (nx > FT ? nx : FT);
-  static constexpr int tw_bytes = TwSize<nx>::total * 16;
+  static constexpr int tw_bytes = TwSize<nx, 16>::total * 16;
   static constexpr int work = (Padded<M>::size * 16 > tmp_n * 8) ? Padded<M>::size * 16 : tmp_n * 8;
   // [twiddles | okx (nx) | rho (nx) | fft buffer / reduction scratch | row_sum scratch | flag]
   static constexpr int lds = tw_bytes + 2 * nx * 8 + work + 64 + 16;
@@ -463,7 +463,7 @@ VP_HD void field_body(Ctx& c, unsigned char* lds, const FieldArgs& a) {
   tw_stage<double, Cfg::nx>(c, twl, a.tw);
   for (int x = c.tid; x < Cfg::nx; x += FT) okx[x] = a.okx[x];
   c.sync();
-  const Tw<double, TwSize<Cfg::nx>::full> tw{twl};
+  const Tw<double, TwSize<Cfg::nx, 16>::full> tw{twl};
   const bool act = c.tid < TPT;
   const int t = act ? c.tid : 0;
   cx<double> v[Q];
@@ -569,7 +569,7 @@ struct DriftCfg {
   // on different LDS banks (an even multiple of 64 banks made the columns a
   // wave touches collide)
   static constexpr int stride = Padded<M>::size | 1;
-  static constexpr int tw_bytes = (TwSize<2 * M>::total * (int)sizeof(cx<T>) + 15) & ~15;
+  static constexpr int tw_bytes = (TwSize<2 * M, (int)sizeof(cx<T>)>::total * (int)sizeof(cx<T>) + 15) & ~15;
   static constexpr int own = tw_bytes + C * stride * (int)sizeof(cx<T>);
   // fused field tail (small grids): M-point Poisson transform on the same workgroup
   static constexpr int fq = M < 8 ? M : 8;
@@ -591,7 +591,7 @@ VP_HD void drift_body(Ctx& c, unsigned char* lds, const DriftArgs<T>& a) {
   const size_t foff = (size_t)inst * (size_t)a.nx * nv;
   const double dk = a.dk_inst ? a.dk_inst[inst] : a.dk;
   tw_stage<T, 2 * M>(c, twl, a.tw);
-  const Tw<T, TwSize<2 * M>::full> tw{twl};
+  const Tw<T, TwSize<2 * M, (int)sizeof(cx<T>)>::full> tw{twl};
   double acc[2 * Q];
 #pragma unroll
   for (int i = 0; i < 2 * Q; ++i) acc[i] = 0.0;

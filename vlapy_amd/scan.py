@@ -81,6 +81,13 @@ class _Class:
                                                       ctypes.byref(t)))
         return t.as_dict()
 
+    def set_tuning(self, t):
+        """Launch shapes as returned by autotune()."""
+        tu = _lib.Tuning()
+        tu.row_q, tu.drift_tile, tu.fuse_field = int(t["row_q"]), int(t["drift_tile"]), 0
+        tu.drift_order, tu.drift_tiles = int(t["drift_order"]), int(t["drift_tiles"])
+        _lib.check(_lib.library().vpfp_batch_set_tuning(self.handle, ctypes.byref(tu)))
+
     def step(self, n_steps, series, stream):
         _lib.check(_lib.library().vpfp_batch_step_n(
             self.handle, ctypes.c_void_p(self.f.data_ptr()), ctypes.c_void_p(self.e.data_ptr()), int(n_steps),
@@ -122,6 +129,33 @@ class Scan:
         for c, s in zip(self.classes, self.streams):
             out[(c.nx, c.nv)] = c.autotune(n_steps, s)
         return out
+
+    def capture(self, n_steps):
+        """Record n_steps of every size class (each class's launches on its own
+        stream, forked from and joined back into the capture stream) into one
+        HIP graph.  replay() then advances every instance n_steps with one
+        graph launch; results equal run(n_steps, record=False) bit for bit.
+        Call after autotune(): the graph keeps the launch shapes it recorded."""
+        self._graph = None
+        g = torch.cuda.CUDAGraph()
+        cap = torch.cuda.Stream()
+        cap.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.graph(g, stream=cap, capture_error_mode="relaxed"):
+            for c, s in zip(self.classes, self.streams):
+                s.wait_stream(cap)
+                c.step(n_steps, None, s)
+            for s in self.streams:
+                cap.wait_stream(s)
+        torch.cuda.current_stream().wait_stream(cap)
+        self._graph = (g, int(n_steps))
+        return self._graph[1]
+
+    def replay(self, times=1):
+        """Launch the captured graph `times` times on the current stream."""
+        if getattr(self, "_graph", None) is None:
+            raise RuntimeError("no graph captured (call capture(n_steps) first)")
+        for _ in range(int(times)):
+            self._graph[0].replay()
 
     def run(self, n_steps, record=True):
         """Advance every instance n_steps.  Returns <E^2>(t) as [n_steps, n_inst]
