@@ -173,55 +173,85 @@ VP_HD void field_density(Ctx& c, const FieldArgs& a, int x0, int XS, double* rho
 // the direct DFT of field_dft_body (small single grids, nx <= kProNx):
 //   E(x) = Re sum_m F_m exp(2 pi i m x / nx) / nx + driver(x),
 //   F_m = i g_m sum_j rho_j exp(-2 pi i m j / nx), g_m = (okx_m - okx_{nx-m}) / 2.
-// Writes E of rows [row0, row0 + nrows) to e_out (and e_out2); the workgroup
-// holding row 0 also forms all E for the series.  All FT threads call it.
-// lds: rho [kProNx] | F [kProNx] complex | tmp [max(2 kProNx, FT)] | erow [nrows]
+// Every workgroup forms all nx values of E (each sum split over P = FT/nx
+// adjacent lanes, joined by shuffles), writes those of rows [row0, row0 +
+// nrows) to e_out (and e_out2); the workgroup holding row 0 writes the
+// series.  All FT threads call it; erow (= lds + ProCfg::erow_off) holds E.
+// lds: tw [kProNx] cx | okx [kProNx] | rho [kProNx] | F [kProNx] cx | tmp [tmp_n] | E [kProNx]
 constexpr int kProNx = 128;
 template <int FT>
 struct ProCfg {
   static constexpr int tmp_n = 2 * kProNx > FT ? 2 * kProNx : FT;
-  static constexpr int bytes = kProNx * 8 + kProNx * 16 + tmp_n * 8 + 64 * 8;
+  static constexpr int tw_off = 0;
+  static constexpr int okx_off = tw_off + kProNx * 16;
+  static constexpr int rho_off = okx_off + kProNx * 8;
+  static constexpr int f_off = rho_off + kProNx * 8;
+  static constexpr int tmp_off = f_off + kProNx * 16;
+  static constexpr int erow_off = tmp_off + tmp_n * 8;
+  static constexpr int bytes = erow_off + kProNx * 8;
 };
 
 template <int FT, class Ctx>
 VP_HD void field_rows(Ctx& c, const FieldArgs& a, unsigned char* lds, int row0, int nrows) {
+  using P_ = ProCfg<FT>;
   const int nx = a.nx;
-  double* rho = reinterpret_cast<double*>(lds);
-  cx<double>* F = reinterpret_cast<cx<double>*>(lds + kProNx * 8);
-  double* tmp = reinterpret_cast<double*>(lds + kProNx * 24);
-  double* erow = tmp + ProCfg<FT>::tmp_n;
-  field_density<FT>(c, a, 0, nx, rho, tmp);
-  for (int m = c.tid; m < nx; m += FT) {
+  cx<double>* twl = reinterpret_cast<cx<double>*>(lds + P_::tw_off);
+  double* okx = reinterpret_cast<double*>(lds + P_::okx_off);
+  double* rho = reinterpret_cast<double*>(lds + P_::rho_off);
+  cx<double>* F = reinterpret_cast<cx<double>*>(lds + P_::f_off);
+  double* tmp = reinterpret_cast<double*>(lds + P_::tmp_off);
+  double* E = reinterpret_cast<double*>(lds + P_::erow_off);
+  for (int i = c.tid; i < nx; i += FT) {
+    twl[i] = a.tw[i];
+    okx[i] = a.okx[i];
+  }
+  field_density<FT>(c, a, 0, nx, rho, tmp);  // ends with a barrier
+  // P lanes per output: the largest power of two <= FT / nx, at most 64
+  int P = 1;
+  while (2 * P * nx <= FT && P < 64) P *= 2;
+  const int part = c.tid % P;
+  const int G = FT / P;                  // outputs per round
+  const int rounds = (nx + G - 1) / G;   // the same for every thread
+  for (int rd = 0; rd < rounds; ++rd) {
+    const int m0 = c.tid / P + rd * G;
+    const int m = m0 < nx ? m0 : nx - 1;
     cx<double> s = mk<double>(0.0, 0.0);
-    for (int j = 0; j < nx; ++j) {
-      const cx<double> w = a.tw[(int)(((long long)m * j) % nx)];
+    for (int j = part; j < nx; j += P) {
+      const cx<double> w = twl[(int)(((long long)m * j) % nx)];
       s.re += rho[j] * w.re;
       s.im += rho[j] * w.im;
     }
-    const double g = 0.5 * (a.okx[m] - a.okx[(nx - m) % nx]);
-    F[m] = mk<double>(-g * s.im, g * s.re);
+    for (int o = 1; o < P; o <<= 1) {
+      s.re += c.shfl_xor(s.re, o, P);
+      s.im += c.shfl_xor(s.im, o, P);
+    }
+    if (part == 0 && m0 < nx) {
+      const double g = 0.5 * (okx[m] - okx[(nx - m) % nx]);
+      F[m] = mk<double>(-g * s.im, g * s.re);
+    }
   }
   c.sync();
-  auto ex = [&](int x) {
+  for (int rd = 0; rd < rounds; ++rd) {
+    const int x0 = c.tid / P + rd * G;
+    const int x = x0 < nx ? x0 : nx - 1;
     double s = 0.0;
-    for (int m = 0; m < nx; ++m) {
-      const cx<double> w = a.tw[(int)(((long long)m * x) % nx)];
+    for (int m = part; m < nx; m += P) {
+      const cx<double> w = twl[(int)(((long long)m * x) % nx)];
       s += F[m].re * w.re + F[m].im * w.im;
     }
-    return s / nx + (a.driver ? a.driver[x] : 0.0);
-  };
+    for (int o = 1; o < P; o <<= 1) s += c.shfl_xor(s, o, P);
+    if (part == 0 && x0 < nx) E[x] = s / nx + (a.driver ? a.driver[x] : 0.0);
+  }
+  c.sync();
   for (int r = c.tid; r < nrows; r += FT) {
-    const double e = ex(row0 + r);
-    erow[r] = e;
-    a.e_out[row0 + r] = e;
-    if (a.e_out2) a.e_out2[row0 + r] = e;
+    a.e_out[row0 + r] = E[row0 + r];
+    if (a.e_out2) a.e_out2[row0 + r] = E[row0 + r];
   }
   if (row0 == 0 && a.series) {
     double acc[2] = {0.0, 0.0};
     for (int x = c.tid; x < nx; x += FT) {
-      const double e = ex(x);
       const double dr = a.driver ? a.driver[x] : 0.0;
-      acc[0] += e * e;
+      acc[0] += E[x] * E[x];
       acc[1] += dr * dr;
     }
     row_sum<FT, 2>(c, acc, tmp);
@@ -306,7 +336,7 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
     const int row0 = c.bid * Cfg::rows;
     const int nr = a.nx - row0 < Cfg::rows ? a.nx - row0 : Cfg::rows;
     field_rows<Cfg::wg>(c, a.fld, lds, row0, nr);
-    e = reinterpret_cast<const double*>(lds + kProNx * 24)[ProCfg<Cfg::wg>::tmp_n + (valid ? rl : 0)];
+    e = reinterpret_cast<const double*>(lds + ProCfg<Cfg::wg>::erow_off)[valid ? row : 0];
     c.sync();  // every thread has its E before the twiddles overwrite the prologue's LDS
   } else if (a.flags & (RK_KICK_A | RK_KICK_B)) {
     e = a.e[rowc];
