@@ -127,3 +127,16 @@ def test_arrays_for_inner_loop_schema():
     b = outer_loop.get_arrays_for_inner_loop(s, 5, {"time": "first-last", "space": ["k0", "k1"]})
     assert b["stored_f"].shape == (5, 2, 16) and b["stored_f"].dtype == np.complex64
     assert set(a["series"]) == set(O.SERIES_KEYS) and set(a["fields"]) == set(O.FIELD_KEYS)
+
+
+def test_bench_refuses_several_gpus():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and says
+    why, before touching a GPU."""
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    res = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=300)
+    assert res.returncode == 2
+    assert "one GPU" in res.stderr
