@@ -124,3 +124,28 @@ def test_conservation_projection():
     net.apply(g)
     np.testing.assert_allclose(g.cpu().numpy(), f + np.float32(0.01) * s.astype(np.float32) * y.astype(np.float32),
                                rtol=0, atol=1e-6)
+
+
+def test_closure_hook_refuses_unsupported_batches():
+    """The batch hook takes FP32 batches with nv = 256 only (NotImplementedError
+    otherwise), and detaches with None."""
+    import ctypes
+
+    from vlapy_amd import _lib
+    from vlapy_amd.closure import Closure
+    from vlapy_amd.scan import Scan
+
+    net = Closure(max_rows=8)
+    lib = _lib.library()
+    p = {"k": np.array([0.3]), "amp": np.array([1e-2]), "nu": np.array([1e-3]), "nx": np.array([32]),
+         "nv": np.array([256])}
+    for precision, nv, ok in (("fp64", 256, False), ("fp32", 512, False), ("fp32", 256, True)):
+        q = dict(p, nv=np.array([nv]))
+        sc = Scan(q, dt=0.1, op="dg", precision=precision)
+        h = sc.classes[0].handle
+        if ok:
+            _lib.check(lib.vpfp_batch_set_closure(h, net.handle))
+            _lib.check(lib.vpfp_batch_set_closure(h, None))
+        else:
+            with pytest.raises(NotImplementedError):
+                _lib.check(lib.vpfp_batch_set_closure(h, net.handle))

@@ -89,3 +89,18 @@ def test_scan_graph_replay_matches_eager():
         for ce, cg in zip(eager.classes, graph.classes):
             assert torch.equal(ce.f, cg.f), (r, ce.nx, ce.nv)
             assert torch.equal(ce.e, cg.e), (r, ce.nx, ce.nv)
+
+
+def test_scan_lenard_bernstein_matches_oracle():
+    """The scan with the Lenard-Bernstein operator (op "lb"), FP64: every
+    instance's <E^2> history within 1e-11 of the oracle's single-instance run."""
+    from vlapy_amd.scan import Scan
+
+    p = _params()
+    n, dt = 10, 0.1
+    sc = Scan(p, dt=dt, op="lb", precision="fp64")
+    e2 = sc.run(n)
+    for i in range(p["k"].size):
+        _, e2_o = _oracle(p, i, n, dt, op="lb")
+        err = np.abs(e2[:, i] - e2_o).max() / np.abs(e2_o).max()
+        assert err < 1e-11, (i, err)
