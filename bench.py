@@ -230,17 +230,17 @@ def scan_window(p, i, e2, t):
     return 6.0, min(0.5 * t_trap, 0.5 * t_rec, t_noise, t[-1])
 
 
-def scan_run(precision, steps):
+def scan_run(precision, steps, delta=False):
     import torch
 
     from vlapy_amd.scan import Scan, draw_parameters
 
     p = draw_parameters(SCAN_N, SCAN_SEED)
-    sc = Scan(p, dt=SCAN_DT, op="dg", precision=precision)
+    sc = Scan(p, dt=SCAN_DT, op="dg", precision=precision, delta=delta)
     sc.run(2, record=False)  # warmup (also first-call kernel setup)
     torch.cuda.synchronize()
     del sc
-    sc = Scan(p, dt=SCAN_DT, op="dg", precision=precision)
+    sc = Scan(p, dt=SCAN_DT, op="dg", precision=precision, delta=delta)
     tuning = sc.autotune(3)  # launch shapes per size class (untimed setup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -250,7 +250,7 @@ def scan_run(precision, steps):
     # the same steps replayed from one HIP graph of SCAN_GRAPH_STEPS steps
     # (from the initial state again; no <E^2> record)
     graph_wall = None
-    if steps % SCAN_GRAPH_STEPS == 0:
+    if steps % SCAN_GRAPH_STEPS == 0 and not delta:
         del sc
         sc = Scan(p, dt=SCAN_DT, op="dg", precision=precision)
         for c, (k, t) in zip(sc.classes, tuning.items()):
@@ -313,17 +313,28 @@ def scan_section(steps, with_cpu):
 
     p, cells, e2_32, wall32, tun32, gwall32 = scan_run("fp32", steps)
     _, _, e2_64, wall64, _, _ = scan_run("fp64", steps)
+    _, _, e2_d, wall_d, _, _ = scan_run("fp32", steps, delta=True)
     t = SCAN_DT * (np.arange(steps) + 1)
-    errs = []
-    for i in range(SCAN_N):
-        lo, hi = scan_window(p, i, e2_64, t)
-        try:
-            g64, _ = rate(e2_64[:, i], t, lo, hi)
-            g32, _ = rate(e2_32[:, i], t, lo, hi)
-        except ValueError:
-            continue
-        errs.append(abs(g32 / g64 - 1.0))
-    errs = np.asarray(errs)
+
+    def damping_errs(e2):
+        errs = []
+        for i in range(SCAN_N):
+            lo, hi = scan_window(p, i, e2_64, t)
+            try:
+                g64, _ = rate(e2_64[:, i], t, lo, hi)
+                g32, _ = rate(e2[:, i], t, lo, hi)
+            except ValueError:
+                continue
+            errs.append(abs(g32 / g64 - 1.0))
+        return np.asarray(errs)
+
+    def summary(errs):
+        return {"instances_fitted": int(errs.size),
+                "within_1pct": float(np.mean(errs <= 0.01)) if errs.size else None,
+                "median_rel_err": float(np.median(errs)) if errs.size else None,
+                "p95_rel_err": float(np.percentile(errs, 95)) if errs.size else None}
+
+    errs = damping_errs(e2_32)
     out = {
         "workload": "4096-instance ragged scan, seed 20261202, Dougherty, dt=0.1, %d steps" % steps,
         "instances_per_s": SCAN_N / wall32,
@@ -339,12 +350,10 @@ def scan_section(steps, with_cpu):
                      "fp64_achieved": 64.0 * cells * steps / wall64 / 1e9,
                      "fp64_frac": 64.0 * cells * steps / wall64 / 1e9 / HBM_PEAK_GBS},
         "tuning": {"%dx%d" % k: v for k, v in tun32.items()},
-        "damping_rate": {
-            "instances_fitted": int(errs.size),
-            "within_1pct": float(np.mean(errs <= 0.01)) if errs.size else None,
-            "median_rel_err": float(np.median(errs)) if errs.size else None,
-            "p95_rel_err": float(np.percentile(errs, 95)) if errs.size else None,
-        },
+        "damping_rate": summary(errs),
+        # FP32 storage of f - M (vpfp_batch_set_delta): damping rates against FP64
+        "delta_f": {"instances_per_s": SCAN_N / wall_d, "seconds": wall_d,
+                    "damping_rate": summary(damping_errs(e2_d))},
     }
     if with_cpu:
         c = scan_cpu(p)

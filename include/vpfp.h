@@ -150,6 +150,19 @@ int vpfp_batch_create(const vpfp_batch_desc* desc, vpfp_batch_t* out);
 int vpfp_batch_step_n(vpfp_batch_t batch, void* f, double* e, int n_steps, double* series, void* stream);
 int vpfp_batch_destroy(vpfp_batch_t batch);
 
+/* Delta-f storage (no counterpart in the reference, which stores f itself).
+ * With on = 1, the f arrays of vpfp_batch_step_n hold f - M(v), M the unit
+ * Maxwellian exp(-v^2/2)/sqrt(2 pi) formed in FP32 on the device: the kicks
+ * add M(v - a) - M(v) = M(v) expm1(a v - a^2/2) to the shifted deviation, the
+ * collision and its moments see M + (f - M) in FP64, and the drift and the
+ * field (whose k = 0 mode is dropped) see the deviation alone.  For FP32
+ * storage this keeps the rounding of f relative to the deviation instead of
+ * to M; the algorithm is unchanged.  Not with an attached closure.
+ * vpfp_batch_background adds sign * M(v) (sign 1 or -1) to every row of a
+ * [n_inst][nx][nv] array: -1 converts f to the stored deviation, 1 back. */
+int vpfp_batch_set_delta(vpfp_batch_t batch, int on);
+int vpfp_batch_background(vpfp_batch_t batch, void* f, int sign, void* stream);
+
 /* Launch shapes of the step and the autotuner that picks them.  No
  * counterpart in the reference (its NumPy path has no launch shapes); the
  * choices change rounding only, never the algorithm.

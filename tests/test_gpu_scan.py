@@ -104,3 +104,57 @@ def test_scan_lenard_bernstein_matches_oracle():
         _, e2_o = _oracle(p, i, n, dt, op="lb")
         err = np.abs(e2[:, i] - e2_o).max() / np.abs(e2_o).max()
         assert err < 1e-11, (i, err)
+
+
+@pytest.mark.parametrize("precision,tol_e2,tol_f", [("fp64", 1e-11, 1e-12), ("fp32", 2e-3, 2e-5)])
+def test_scan_delta_f_matches_oracle(precision, tol_e2, tol_f):
+    """Delta-f storage (f - M stored, vpfp_batch_set_delta) runs the same
+    algorithm: FP64 within the plain scan's 1e-11 of the oracle, f (the
+    deviation plus the background, vpfp_batch_background) within 1e-12; FP32
+    within the plain FP32 scan's bounds."""
+    from vlapy_amd.scan import Scan
+
+    p = _params()
+    n, dt = 20, 0.1
+    sc = Scan(p, dt=dt, op="dg", precision=precision, delta=True)
+    e2 = sc.run(n)
+    for i in range(p["k"].size):
+        f_o, e2_o = _oracle(p, i, n, dt)
+        err = np.abs(e2[:, i] - e2_o).max() / np.abs(e2_o).max()
+        print(precision, "delta-f instance", i, "E^2 rel err", err)
+        assert err < tol_e2, (i, err)
+        c = [c for c in sc.classes if i in c.idx][0]
+        j = int(np.nonzero(c.idx == i)[0][0])
+        f_g = c.full_f()[j].double().cpu().numpy()
+        assert np.abs(f_g - f_o).max() / np.abs(f_o).max() < tol_f, i
+    # the stored deviation is small next to f itself
+    c = sc.classes[0]
+    assert float(c.f.abs().max()) < 0.2 * float(c.full_f().abs().max())
+
+
+def test_scan_delta_f_refuses_closure_and_round_trips_background():
+    import ctypes
+
+    import torch
+
+    from vlapy_amd import _lib
+    from vlapy_amd.closure import Closure
+    from vlapy_amd.scan import Scan
+
+    p = {key: val[:2] for key, val in _params().items()}
+    sc = Scan(p, dt=0.1, precision="fp32", delta=True)
+    c = sc.classes[0]
+    net = Closure(max_rows=16)
+    rc = _lib.library().vpfp_batch_set_closure(c.handle, net.handle)
+    assert rc == _lib.VPFP_EUNSUPPORTED
+    # FP64 batch: background added then removed leaves the array to rounding
+    sc64 = Scan(p, dt=0.1, precision="fp64", delta=False)
+    c64 = sc64.classes[0]
+    f = c64.f.clone()
+    lib = _lib.library()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    _lib.check(lib.vpfp_batch_background(c64.handle, ctypes.c_void_p(f.data_ptr()), -1, st))
+    assert float(f.abs().max()) < 0.01 * float(c64.f.abs().max()) + 1e-6
+    _lib.check(lib.vpfp_batch_background(c64.handle, ctypes.c_void_p(f.data_ptr()), 1, st))
+    torch.cuda.synchronize()
+    assert float((f - c64.f).abs().max()) < 1e-15

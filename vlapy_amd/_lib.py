@@ -90,6 +90,8 @@ SIGNATURES = {
     "vpfp_batch_create": [ctypes.POINTER(BatchDesc), ctypes.POINTER(_c_p)],
     "vpfp_batch_step_n": [_c_p, _c_p, _c_p, _c_i, _c_p, _c_p],
     "vpfp_batch_destroy": [_c_p],
+    "vpfp_batch_set_delta": [_c_p, _c_i],
+    "vpfp_batch_background": [_c_p, _c_p, _c_i, _c_p],
     "vpfp_plan_set_profiling": [_c_p, _c_i],
     "vpfp_plan_kernel_times": [_c_p, ctypes.POINTER(_c_d), ctypes.POINTER(ctypes.c_longlong)],
     "vpfp_plan_get_tuning": [_c_p, ctypes.POINTER(Tuning)],

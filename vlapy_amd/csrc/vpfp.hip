@@ -51,11 +51,11 @@ struct GpuCtx {
 #define VPFP_CTX GpuCtx c{(int)threadIdx.x, (int)blockDim.x, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y}
 
 // ------------------------------------------------------------------ kernels
-template <class T, int M, int Q>
+template <class T, int M, int Q, bool BG = false>
 __global__ __launch_bounds__((vp::RowCfg<T, M, Q>::wg), (vp::RowCfg<T, M, Q>::min_waves)) void k_row(vp::RowArgs<T> a) {
   extern __shared__ __align__(16) unsigned char lds[];
   VPFP_CTX;
-  vp::row_body<T, M, Q>(c, lds, a);
+  vp::row_body<T, M, Q, GpuCtx, BG>(c, lds, a);
 }
 
 template <class T, int M, int Q, int C>
@@ -80,6 +80,15 @@ __global__ __launch_bounds__(kFT) void k_field_dft(vp::FieldArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   VPFP_CTX;
   vp::field_dft_body<kFT>(c, lds, a);
+}
+
+// f[r][m] += sign * bg[m] on every row (delta-f storage conversion)
+template <class T>
+__global__ __launch_bounds__(256) void k_background(T* f, size_t n, int nv, const double* bg, double sign) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int m = (int)(i % (size_t)nv);
+  f[i] = (T)((double)f[i] + sign * bg[m]);
 }
 
 template <class T>
@@ -142,10 +151,10 @@ int drift_wgs(int nv, int C, int per, bool batch) {
 // field workgroups: one per 64 x values (the last to finish solves Poisson)
 int field_grid(int nx) { return nx >= 128 ? nx / 64 : 1; }
 
-template <class T, int M, int Q>
+template <class T, int M, int Q, bool BG = false>
 int launch_row_q(const vp::RowArgs<T>& a, hipStream_t s) {
   using Cfg = vp::RowCfg<T, M, Q>;
-  auto kern = k_row<T, M, Q>;
+  auto kern = k_row<T, M, Q, BG>;
   if (int rc = set_lds(kern, Cfg::lds)) return rc;
   const int grid = (a.nx + Cfg::rows - 1) / Cfg::rows;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(Cfg::wg), Cfg::lds, s, a);
@@ -159,6 +168,8 @@ int launch_row_q(const vp::RowArgs<T>& a, hipStream_t s) {
 // 16 <= M <= 2048, 16 for M >= 256), 0 = the row-count rule above
 template <class T, int M>
 int launch_row(const vp::RowArgs<T>& a, hipStream_t s, int q) {
+  // delta-f storage: one launch shape
+  if (a.bg_hat) return launch_row_q<T, M, qfor(M), true>(a, s);
   if constexpr (M >= 256 && M <= 4096) {
     if (q == 16) return launch_row_q<T, M, 16>(a, s);
   }
@@ -652,6 +663,9 @@ struct vpfp_batch_s {
   int dtiles = 1;
   double tuned_us = 0.0;
   vpfc_net_t closure = nullptr;  // learned closure after every step (FP32, nv 256)
+  int delta = 0;                 // f stored as f - M(v)
+  double* bg = nullptr;          // M(v_m) [nv] and its unnormalised DFT [nv]
+  cx<double>* bg_hat = nullptr;
   double dv = 0, dt = 0, v0 = 0, dvv = 0, dk_v = 0;
   double *dk_x = nullptr, *okx = nullptr, *nu_dt = nullptr, *v = nullptr;
   void *tw_v = nullptr, *tw_x = nullptr;
@@ -711,6 +725,10 @@ int batch_step_n(vpfp_batch_s* b, void* f, double* e, int n, double* series, hip
   ra.op_dg = b->op == VPFP_OP_DG;
   ra.nu_dt_inst = b->nu_dt;
   ra.rows_per_inst = b->nx;
+  if (b->delta) {
+    ra.bg = b->bg;
+    ra.bg_hat = b->bg_hat;
+  }
   ra.dt_a = ra.dt_b = 0.5 * b->dt;
   ra.e = e;
   ra.flags = vp::RK_KICK_A;
@@ -1200,7 +1218,68 @@ int vpfp_batch_set_closure(vpfp_batch_t b, vpfc_net_t net) {
   if (!b) return fail(VPFP_EBADARG, "null batch");
   if (net && (b->prec != VPFP_F32 || b->nv != VPFC_NV))
     return fail(VPFP_EUNSUPPORTED, "the closure runs on FP32 batches with nv = " + std::to_string(VPFC_NV));
+  if (net && b->delta) return fail(VPFP_EUNSUPPORTED, "the closure reads f itself: not with delta-f storage");
   b->closure = net;
+  return VPFP_OK;
+}
+
+namespace {
+
+// unit Maxwellian on the batch's v grid and its DFT sum_m M_m exp(-2 pi i k m / N)
+// (direct sum in FP64 with an exact twiddle index, once per batch)
+int batch_background(vpfp_batch_s* b) {
+  if (b->bg) return VPFP_OK;
+  const int N = b->nv;
+  std::vector<double> m(N);
+  for (int i = 0; i < N; ++i) {
+    const double v = b->v0 + i * b->dvv;
+    m[i] = std::exp(-0.5 * v * v) / std::sqrt(6.283185307179586);
+  }
+  std::vector<double> wr(N), wi(N);
+  for (int j = 0; j < N; ++j) {
+    wr[j] = std::cos(6.283185307179586 * j / N);
+    wi[j] = -std::sin(6.283185307179586 * j / N);
+  }
+  std::vector<cx<double>> hat(N);
+  for (int k = 0; k < N; ++k) {
+    double re = 0.0, im = 0.0;
+    for (int i = 0; i < N; ++i) {
+      const int j = (int)(((long long)k * i) % N);
+      re += m[i] * wr[j];
+      im += m[i] * wi[j];
+    }
+    hat[k] = vp::mk<double>(re, im);
+  }
+  if (int rc = bput(b, &b->bg, m)) return rc;
+  return bput(b, &b->bg_hat, hat);
+}
+
+}  // namespace
+
+int vpfp_batch_set_delta(vpfp_batch_t b, int on) {
+  if (!b) return fail(VPFP_EBADARG, "null batch");
+  if (on != 0 && on != 1) return fail(VPFP_EBADARG, "delta must be 0 or 1");
+  if (on && b->closure) return fail(VPFP_EUNSUPPORTED, "the closure reads f itself: not with delta-f storage");
+  if (on)
+    if (int rc = batch_background(b)) return rc;
+  b->delta = on;
+  return VPFP_OK;
+}
+
+int vpfp_batch_background(vpfp_batch_t b, void* f, int sign, void* stream) {
+  if (!b) return fail(VPFP_EBADARG, "null batch");
+  if (!f) return fail(VPFP_EBADARG, "null array");
+  if (sign != 1 && sign != -1) return fail(VPFP_EBADARG, "sign must be 1 or -1");
+  if (int rc = batch_background(b)) return rc;
+  const size_t n = (size_t)b->n * b->nx * b->nv;
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  if (b->prec == VPFP_F64)
+    hipLaunchKernelGGL(k_background<double>, dim3(grid), dim3(256), 0, S(stream), reinterpret_cast<double*>(f), n,
+                       b->nv, b->bg, (double)sign);
+  else
+    hipLaunchKernelGGL(k_background<float>, dim3(grid), dim3(256), 0, S(stream), reinterpret_cast<float*>(f), n,
+                       b->nv, b->bg, (double)sign);
+  VPFP_HIP(hipGetLastError());
   return VPFP_OK;
 }
 

@@ -287,6 +287,7 @@ VP_HD void fft(Ctx& c, cx<T>* v, cx<T>* buf, const TW& tw, int t, bool act = tru
 // phase is P_{M-k} = exp(-i theta_M) conj(P_k).
 template <class T>
 struct PhaseMul {
+  static constexpr bool kBg = false;
   cx<T> cur, step, pm;
   VP_HD PhaseMul(double phi, int M, int t, int TPT) {
     cis_neg((double)M * phi, pm);
@@ -307,8 +308,51 @@ struct PhaseMul {
 
 // Poisson multiplier i * one_over_kx (zero real part at DC and Nyquist);
 // okx is an LDS copy of the one_over_kx table.
+// exp(-i theta) - 1 without cancellation: (-2 sin^2(theta/2), -sin theta)
+template <class T>
+VP_HD cx<T> cis_neg_m1(double th) {
+  const double r = th - 6.283185307179586 * rint(th * 0.15915494309189535);
+  const double h = sin(0.5 * r);
+  return mk<T>((T)(-2.0 * h * h), (T)(-sin(r)));
+}
+
+// PhaseMul on a stored deviation f - B from a fixed background B whose
+// unnormalised spectrum is bhat [N]: the shift of B itself enters as
+// F'_k += bhat_k (P_k - 1), F'_{k+M} += bhat_{k+M} (conj P_{M-k} - 1), with
+// c_k = P_k - 1 carried by c_{k+s} = c_k + c_s + c_k c_s (no cancellation
+// for small shifts).  couple() calls bg_add(k, ...) right after pair(k, ...).
+template <class T>
+struct BgPhaseMul {
+  static constexpr bool kBg = true;
+  PhaseMul<T> ph;
+  cx<T> cm, cs, cpm;  // P_k - 1, P_TPT - 1, exp(+i M phi) - 1
+  const cx<double>* bhat;
+  int M;
+  VP_HD BgPhaseMul(double phi, int M_, int t, int TPT, const cx<double>* bhat_)
+      : ph(phi, M_, t, TPT), bhat(bhat_), M(M_) {
+    cm = cis_neg_m1<T>((double)t * phi);
+    cs = cis_neg_m1<T>((double)TPT * phi);
+    cpm = cis_neg_m1<T>(-(double)M_ * phi);
+  }
+  VP_HD void pair(int k, cx<T>& pk, cx<T>& pmk) { ph.pair(k, pk, pmk); }
+  VP_HD void bg_add(int k, cx<T>& G1, cx<T>& G2) {
+    const cx<double> b1 = bhat[k], b2 = bhat[k + M];
+    const cx<T> B1 = mk<T>((T)b1.re, (T)b1.im), B2 = mk<T>((T)b2.re, (T)b2.im);
+    if (k == 0) {
+      // DC unchanged; Nyquist multiplier Re(P_M) = cos(M phi)
+      G2 = add(G2, mk<T>(B2.re * cpm.re, B2.im * cpm.re));
+    } else {
+      G1 = add(G1, mul(B1, cm));
+      const cx<T> q = add(add(cpm, cm), mul(cpm, cm));
+      G2 = add(G2, mul(B2, q));
+    }
+    cm = add(add(cm, cs), mul(cm, cs));
+  }
+};
+
 template <class T>
 struct PoissonMul {
+  static constexpr bool kBg = false;
   const double* okx;
   int M;
   VP_HD void pair(int k, cx<T>& pk, cx<T>& pmk) {
@@ -345,8 +389,9 @@ VP_HD void couple(cx<T>* v, const cx<T>* pz, const TW& tw, int t, Mul& mult) {
     const cx<T> Fk = add(E, WO), Fkm = sub(E, WO);
     cx<T> pk, pmk;
     mult.pair(k, pk, pmk);
-    const cx<T> G1 = mul(pk, Fk);
-    const cx<T> G2 = mulc(Fkm, pmk);
+    cx<T> G1 = mul(pk, Fk);
+    cx<T> G2 = mulc(Fkm, pmk);
+    if constexpr (Mul::kBg) mult.bg_add(k, G1, G2);
     const cx<T> S = add(G1, G2);
     const cx<T> D = mulc(sub(G1, G2), W);
     v[i] = mk<T>((S.re - D.im) * invN, (S.im + D.re) * invN);

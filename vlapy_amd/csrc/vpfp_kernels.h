@@ -287,6 +287,12 @@ struct RowArgs {
   // drift's partial densities instead of e (single grid, nx <= kProNx)
   int field_pro;
   FieldArgs fld;
+  // delta-f storage: f holds f - B(v) for a fixed background B (bg [nv],
+  // unnormalised spectrum bg_hat [nv]); the kicks shift B spectrally as part
+  // of the deviation's multiply (BgPhaseMul), the collision and moments see
+  // B + (f - B) in FP64
+  const double* bg;
+  const cx<double>* bg_hat;
 };
 
 template <class T, int M, int Q>
@@ -311,7 +317,9 @@ struct RowCfg {
   static constexpr int min_waves = (Q == 8 && sizeof(T) == 4 && M <= 1024) ? 3 : 1;
 };
 
-template <class T, int M, int Q, class Ctx>
+// BG: delta-f storage (a.bg, a.bg_hat set); a kernel of its own so that the
+// plain step's register allocation is unchanged
+template <class T, int M, int Q, class Ctx, bool BG = false>
 VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
   using Cfg = RowCfg<T, M, Q>;
   constexpr int TPT = Cfg::TPT;
@@ -345,8 +353,15 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
   c.sync();
   const Tw<T, TwSize<2 * M, (int)sizeof(cx<T>)>::full> tw{twl};
 
-  if (a.flags & RK_KICK_A)
-    spectral_apply<T, M, TPT>(c, v, buf, tw, t, PhaseMul<T>((a.dk * a.dt_a) * e, M, t, TPT));
+  // kick by shift a: f(v) -> f(v - a), spectrally (the background's shift too
+  // under delta-f storage)
+  auto kick = [&](double shift) {
+    if constexpr (BG)
+      spectral_apply<T, M, TPT>(c, v, buf, tw, t, BgPhaseMul<T>(a.dk * shift, M, t, TPT, a.bg_hat));
+    else
+      spectral_apply<T, M, TPT>(c, v, buf, tw, t, PhaseMul<T>(a.dk * shift, M, t, TPT));
+  };
+  if (a.flags & RK_KICK_A) kick(a.dt_a * e);
 
   if (a.flags & (RK_COLLIDE | RK_MOMENTS)) {
     // natural order -> contiguous chunk of L samples per thread
@@ -363,6 +378,10 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
     c.sync();
     const int N = a.nv;
     const int m0 = t * L;
+    if constexpr (BG) {
+#pragma unroll
+      for (int i = 0; i < L; ++i) d[i] += a.bg[m0 + i];
+    }
     const double nu_dt = a.nu_dt_inst ? a.nu_dt_inst[rowc / a.rows_per_inst] : a.nu_dt;
     if ((a.flags & RK_COLLIDE) && nu_dt > 0.0) {
       double vbar, temp;
@@ -404,6 +423,10 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
         for (int q = 0; q < kMoments; ++q) a.moments[(size_t)q * a.nx + row] = a.dv * s[q];
       }
     }
+    if constexpr (BG) {
+#pragma unroll
+      for (int i = 0; i < L; ++i) d[i] -= a.bg[m0 + i];
+    }
     // chunk -> natural order
 #pragma unroll
     for (int i = 0; i < Q; ++i) buf[pidx(t * Q + i)] = mk<T>((T)d[2 * i], (T)d[2 * i + 1]);
@@ -413,8 +436,7 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
     c.sync();
   }
 
-  if (a.flags & RK_KICK_B)
-    spectral_apply<T, M, TPT>(c, v, buf, tw, t, PhaseMul<T>((a.dk * a.dt_b) * e, M, t, TPT));
+  if (a.flags & RK_KICK_B) kick(a.dt_b * e);
 
   if (valid && a.f_out) {
     cx<T>* dst = reinterpret_cast<cx<T>*>(a.f_out + (size_t)row * a.nv);

@@ -47,7 +47,7 @@ def instance_grids(nx, k, nv, vmax):
 class _Class:
     """One size class (nx, nv): a C-ABI batch and its device state."""
 
-    def __init__(self, idx, params, vmax, dt, op, precision, dev):
+    def __init__(self, idx, params, vmax, dt, op, precision, dev, delta=False):
         self.idx = idx
         self.n = idx.size
         self.nx, self.nv = int(params["nx"][idx[0]]), int(params["nv"][idx[0]])
@@ -72,8 +72,24 @@ class _Class:
         dens = 1.0 + amp * torch.cos(k * x)
         vt = torch.as_tensor(v, device=dev)
         maxw = torch.exp(-vt ** 2 / 2.0) / np.sqrt(2.0 * np.pi)
-        self.f = (dens[:, :, None] * maxw[None, None, :]).to(_dev.f_dtype(precision)).contiguous()
+        f0 = dens[:, :, None] * maxw[None, None, :]
+        self.delta = bool(delta)
+        if self.delta:
+            # stored deviation f0 - M, formed in FP64 and rounded once to the
+            # storage type (the device's M is the same FP64 Maxwellian)
+            _lib.check(_lib.library().vpfp_batch_set_delta(h, 1))
+            f0 = f0 - maxw[None, None, :]
+        self.f = f0.to(_dev.f_dtype(precision)).contiguous()
         self.e = torch.zeros(self.n, self.nx, dtype=torch.float64, device=dev)
+
+    def full_f(self):
+        """f itself (a copy in the storage type), also under delta-f storage."""
+        f = self.f.clone()
+        if self.delta:
+            _lib.check(_lib.library().vpfp_batch_background(
+                self.handle, ctypes.c_void_p(f.data_ptr()), 1,
+                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return f
 
     def autotune(self, n_steps, stream):
         t = _lib.Tuning()
@@ -106,10 +122,12 @@ class Scan:
     params: per-instance arrays k, amp, nu, nx, nv (see draw_parameters).
     Each instance: x in [0, 2 pi/k), v in [-vmax, vmax], f0 = (1 + amp cos kx)
     times a unit Maxwellian, e0 = 0, then every step the leapfrog split and
-    the implicit collision step (op "dg" or "lb").
+    the implicit collision step (op "dg" or "lb").  delta=True stores f - M
+    (vpfp_batch_set_delta): same algorithm, FP32 rounding relative to the
+    deviation from the Maxwellian instead of to f.
     """
 
-    def __init__(self, params, vmax=6.4, dt=0.1, op="dg", precision="fp32"):
+    def __init__(self, params, vmax=6.4, dt=0.1, op="dg", precision="fp32", delta=False):
         self.params = {key: np.asarray(val) for key, val in params.items()}
         self.n = self.params["k"].size
         self.vmax, self.dt, self.precision = float(vmax), float(dt), precision
@@ -118,7 +136,7 @@ class Scan:
         self.classes = []
         for nx, nv in keys:
             idx = np.nonzero((self.params["nx"] == nx) & (self.params["nv"] == nv))[0]
-            self.classes.append(_Class(idx, self.params, vmax, dt, op, precision, dev))
+            self.classes.append(_Class(idx, self.params, vmax, dt, op, precision, dev, delta))
         self.streams = [torch.cuda.Stream() for _ in self.classes]
         self.cells = int(np.sum(self.params["nx"].astype(np.int64) * self.params["nv"].astype(np.int64)))
 
@@ -181,7 +199,7 @@ class Scan:
         """Box-averaged trapz density of every instance (host)."""
         out = np.zeros(self.n)
         for c in self.classes:
-            f = c.f.double()
+            f = c.full_f().double()
             n = c.dv * (f.sum(-1) - 0.5 * (f[..., 0] + f[..., -1]))
             out[c.idx] = n.mean(-1).cpu().numpy()
         return out
