@@ -665,7 +665,7 @@ struct vpfp_batch_s {
   vpfc_net_t closure = nullptr;  // learned closure after every step (FP32, nv 256)
   int delta = 0;                 // f stored as f - M(v)
   double* bg = nullptr;          // M(v_m) [nv] and its unnormalised DFT [nv]
-  cx<double>* bg_hat = nullptr;
+  void* bg_hat = nullptr;        // (cx<double> or cx<float>: the storage precision)
   double dv = 0, dt = 0, v0 = 0, dvv = 0, dk_v = 0;
   double *dk_x = nullptr, *okx = nullptr, *nu_dt = nullptr, *v = nullptr;
   void *tw_v = nullptr, *tw_x = nullptr;
@@ -727,7 +727,7 @@ int batch_step_n(vpfp_batch_s* b, void* f, double* e, int n, double* series, hip
   ra.rows_per_inst = b->nx;
   if (b->delta) {
     ra.bg = b->bg;
-    ra.bg_hat = b->bg_hat;
+    ra.bg_hat = reinterpret_cast<const cx<T>*>(b->bg_hat);
   }
   ra.dt_a = ra.dt_b = 0.5 * b->dt;
   ra.e = e;
@@ -1251,7 +1251,18 @@ int batch_background(vpfp_batch_s* b) {
     hat[k] = vp::mk<double>(re, im);
   }
   if (int rc = bput(b, &b->bg, m)) return rc;
-  return bput(b, &b->bg_hat, hat);
+  if (b->prec == VPFP_F64) {
+    cx<double>* d = nullptr;
+    if (int rc = bput(b, &d, hat)) return rc;
+    b->bg_hat = d;
+  } else {
+    std::vector<cx<float>> hf(N);
+    for (int k = 0; k < N; ++k) hf[k] = vp::mk<float>((float)hat[k].re, (float)hat[k].im);
+    cx<float>* d = nullptr;
+    if (int rc = bput(b, &d, hf)) return rc;
+    b->bg_hat = d;
+  }
+  return VPFP_OK;
 }
 
 }  // namespace

@@ -317,27 +317,30 @@ VP_HD cx<T> cis_neg_m1(double th) {
 }
 
 // PhaseMul on a stored deviation f - B from a fixed background B whose
-// unnormalised spectrum is bhat [N]: the shift of B itself enters as
-// F'_k += bhat_k (P_k - 1), F'_{k+M} += bhat_{k+M} (conj P_{M-k} - 1), with
-// c_k = P_k - 1 carried by c_{k+s} = c_k + c_s + c_k c_s (no cancellation
-// for small shifts).  couple() calls bg_add(k, ...) right after pair(k, ...).
+// unnormalised spectrum is bhat [N] (storage precision): the shift of B itself
+// enters as F'_k += bhat_k (P_k - 1), F'_{k+M} += bhat_{k+M} (conj P_{M-k} - 1),
+// with c_k = P_k - 1 carried by c_{k+s} = c_k + c_s + c_k c_s (no cancellation
+// for small shifts).  couple() calls init() once, then bg_add(k, ...) right
+// after pair(k, ...); the c state is formed in init() so that it is not live
+// across the forward transform.
 template <class T>
 struct BgPhaseMul {
   static constexpr bool kBg = true;
   PhaseMul<T> ph;
+  double phi;
+  int M, t, TPT;
+  const cx<T>* bhat;
   cx<T> cm, cs, cpm;  // P_k - 1, P_TPT - 1, exp(+i M phi) - 1
-  const cx<double>* bhat;
-  int M;
-  VP_HD BgPhaseMul(double phi, int M_, int t, int TPT, const cx<double>* bhat_)
-      : ph(phi, M_, t, TPT), bhat(bhat_), M(M_) {
+  VP_HD BgPhaseMul(double phi_, int M_, int t_, int TPT_, const cx<T>* bhat_)
+      : ph(phi_, M_, t_, TPT_), phi(phi_), M(M_), t(t_), TPT(TPT_), bhat(bhat_) {}
+  VP_HD void init() {
     cm = cis_neg_m1<T>((double)t * phi);
     cs = cis_neg_m1<T>((double)TPT * phi);
-    cpm = cis_neg_m1<T>(-(double)M_ * phi);
+    cpm = cis_neg_m1<T>(-(double)M * phi);
   }
   VP_HD void pair(int k, cx<T>& pk, cx<T>& pmk) { ph.pair(k, pk, pmk); }
   VP_HD void bg_add(int k, cx<T>& G1, cx<T>& G2) {
-    const cx<double> b1 = bhat[k], b2 = bhat[k + M];
-    const cx<T> B1 = mk<T>((T)b1.re, (T)b1.im), B2 = mk<T>((T)b2.re, (T)b2.im);
+    const cx<T> B1 = bhat[k], B2 = bhat[k + M];
     if (k == 0) {
       // DC unchanged; Nyquist multiplier Re(P_M) = cos(M phi)
       G2 = add(G2, mk<T>(B2.re * cpm.re, B2.im * cpm.re));
@@ -378,6 +381,7 @@ VP_HD void couple(cx<T>* v, const cx<T>* pz, const TW& tw, int t, Mul& mult) {
   constexpr int Q = M / TPT;
   const T invN = T(1) / T(2 * M);
   const T h = T(0.5);
+  if constexpr (Mul::kBg) mult.init();
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const int k = t + i * TPT;

@@ -288,11 +288,11 @@ struct RowArgs {
   int field_pro;
   FieldArgs fld;
   // delta-f storage: f holds f - B(v) for a fixed background B (bg [nv],
-  // unnormalised spectrum bg_hat [nv]); the kicks shift B spectrally as part
-  // of the deviation's multiply (BgPhaseMul), the collision and moments see
-  // B + (f - B) in FP64
+  // FP64; unnormalised spectrum bg_hat [nv] in the storage precision); the
+  // kicks shift B spectrally as part of the deviation's multiply (BgPhaseMul),
+  // the collision and moments see B + (f - B) in FP64
   const double* bg;
-  const cx<double>* bg_hat;
+  const cx<T>* bg_hat;
 };
 
 template <class T, int M, int Q>
