@@ -464,14 +464,14 @@ def closure_section():
     out = {"workload": "%d runs, k ~ U[0.25, 0.45], amplitude 10^U[-4,-1], nu 10^U[-5,-2] (Dougherty), nx=64, "
                        "dt=0.05, %d steps; fine grid nv=4096 FP64 vs coarse nv=256 FP32 with and without the "
                        "closure" % (CLOSURE_N, CLOSURE_STEPS)}
-    runs = (("fine_fp64", CL.NV_FINE, "fp64", None), ("coarse_fp32", CL.NV, "fp32", None),
-            ("coarse_fp32_closure", CL.NV, "fp32", net))
+    runs = (("fine_fp64", CL.NV_FINE, "fp64", None, False), ("coarse_fp32", CL.NV, "fp32", None, False),
+            ("coarse_fp32_closure", CL.NV, "fp32", net, False), ("coarse_fp32_delta_f", CL.NV, "fp32", None, True))
     ref = None
-    for name, nv, prec, n in runs:
-        CL.rollout(CL.draw_instances(4, 99), nv, prec, 2, n)  # warm-up
+    for name, nv, prec, n, delta in runs:
+        CL.rollout(CL.draw_instances(4, 99), nv, prec, 2, n, delta)  # warm-up
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        e2 = CL.rollout(inst, nv, prec, CLOSURE_STEPS, n)
+        e2 = CL.rollout(inst, nv, prec, CLOSURE_STEPS, n, delta)
         torch.cuda.synchronize()
         sec = time.perf_counter() - t0
         r = {"instances_per_s": CLOSURE_N / sec, "seconds": sec,
@@ -487,7 +487,7 @@ def closure_section():
                       "damping_within_1pct": float(np.mean(d[ok] <= 0.01))})
         out[name] = r
     out["speedup_vs_fine_fp64"] = {k: out[k]["instances_per_s"] / out["fine_fp64"]["instances_per_s"]
-                                   for k in ("coarse_fp32", "coarse_fp32_closure")}
+                                   for k in ("coarse_fp32", "coarse_fp32_closure", "coarse_fp32_delta_f")}
     return out
 
 

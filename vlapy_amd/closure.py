@@ -193,21 +193,21 @@ def draw_instances(n, seed):
     return {"k": k, "amp": amp, "nu": nu}
 
 
-def _scan(inst, nv, precision):
+def _scan(inst, nv, precision, delta=False):
     from .scan import Scan
 
     n = inst["k"].size
     p = dict(inst, nx=np.full(n, NX), nv=np.full(n, nv))
-    return Scan(p, vmax=VMAX, dt=DT, op="dg", precision=precision)
+    return Scan(p, vmax=VMAX, dt=DT, op="dg", precision=precision, delta=delta)
 
 
-def background():
+def background(dtype=np.float32):
     """The unit Maxwellian of the fine grid averaged onto the coarse grid
-    (FP32 [256]): the coarse representation of the equilibrium."""
+    ([256], FP32 by default): the coarse representation of the equilibrium."""
     dvf = 2 * VMAX / NV_FINE
     vf = np.linspace(-VMAX + dvf / 2.0, VMAX - dvf / 2.0, NV_FINE)
     m = np.exp(-vf ** 2 / 2.0) / np.sqrt(2.0 * np.pi)
-    return m.reshape(NV, NV_FINE // NV).mean(-1).astype(np.float32)
+    return m.reshape(NV, NV_FINE // NV).mean(-1).astype(dtype)
 
 
 def coarse_v():
@@ -315,19 +315,26 @@ def train(x, t, steps, batch=4096, lr=1e-3, seed=0, net=None, log=None, bg=None,
     return net, hist
 
 
-def rollout(inst, nv, precision, n_steps, net=None):
+def rollout(inst, nv, precision, n_steps, net=None, delta=False):
     """<E^2>(t) [n_steps, n] of plain runs on an nv grid, or of FP32 coarse
     runs with the closure applied after every step (net given).  Coarse runs
     start from the fine initial state averaged onto their grid,
-    (1 + amp cos kx) background(), the projection the closure is trained on."""
-    sc = _scan(inst, nv, precision)
+    (1 + amp cos kx) background(), the projection the closure is trained on.
+    delta: store f - M (vpfp_batch_set_delta); the coarse initial state is
+    then formed from the FP64 cell average."""
+    sc = _scan(inst, nv, precision, delta)
     if nv == NV:
         from .scan import instance_grids
 
         c = sc.classes[0]
         x = np.stack([instance_grids(NX, k, NV, VMAX)[0] for k in inst["k"]])
         dens = 1.0 + inst["amp"][:, None] * np.cos(inst["k"][:, None] * x)
-        f0 = dens[:, :, None] * background().astype(np.float64)[None, None, :]
+        if delta:
+            v = coarse_v()
+            f0 = dens[:, :, None] * background(np.float64)[None, None, :]
+            f0 = f0 - (np.exp(-v ** 2 / 2.0) / np.sqrt(2.0 * np.pi))[None, None, :]
+        else:
+            f0 = dens[:, :, None] * background().astype(np.float64)[None, None, :]
         c.f.copy_(torch.as_tensor(f0, device=c.f.device).to(c.f.dtype))
     if net is not None:
         _lib.check(_lib.library().vpfp_batch_set_closure(sc.classes[0].handle, net.handle))
