@@ -67,6 +67,19 @@ typedef struct {
 int vpfp_plan_create(const vpfp_grid_desc* desc, vpfp_plan_t* out);
 int vpfp_plan_destroy(vpfp_plan_t plan);
 
+/* Delta-f storage for a plan (no counterpart in the reference, which stores
+ * f itself; the batch form and its algorithm are described at
+ * vpfp_batch_set_delta).  With on = 1 the f arrays of vpfp_step_n,
+ * vpfp_edfdv, vpfp_vdfdx, vpfp_collide, vpfp_moments and vpfp_field hold
+ * f - M(v); their results are those of the plain plan on f (moments are of f
+ * itself).  Entry points whose output would change meaning
+ * (vpfp_compute_charges, vpfp_collision_diagonals, vpfp_tridiag_solve,
+ * vpfp_x_modes) return VPFP_EUNSUPPORTED while it is on.  Needs a
+ * power-of-two nv.  vpfp_plan_background adds sign * M(v) (sign 1 or -1) to
+ * every row of an (nx, nv) array. */
+int vpfp_plan_set_delta(vpfp_plan_t plan, int on);
+int vpfp_plan_background(vpfp_plan_t plan, void* f, int sign, void* stream);
+
 /* x-advection f_out = Re ifft_x(exp(-i kx v dt) fft_x f_in).
  * Replaces step_vdfdx_exponential, vlasov.py:94-108.  f_out may equal f_in. */
 int vpfp_vdfdx(vpfp_plan_t plan, const void* f_in, void* f_out, double dt, void* stream);

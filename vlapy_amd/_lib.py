@@ -91,6 +91,8 @@ SIGNATURES = {
     "vpfp_batch_step_n": [_c_p, _c_p, _c_p, _c_i, _c_p, _c_p],
     "vpfp_batch_destroy": [_c_p],
     "vpfp_batch_set_delta": [_c_p, _c_i],
+    "vpfp_plan_set_delta": [_c_p, _c_i],
+    "vpfp_plan_background": [_c_p, _c_p, _c_i, _c_p],
     "vpfp_batch_background": [_c_p, _c_p, _c_i, _c_p],
     "vpfp_plan_set_profiling": [_c_p, _c_i],
     "vpfp_plan_kernel_times": [_c_p, ctypes.POINTER(_c_d), ctypes.POINTER(ctypes.c_longlong)],
@@ -192,6 +194,18 @@ class Plan:
 
     def set_profiling(self, on):
         check(library().vpfp_plan_set_profiling(self.handle, 1 if on else 0))
+
+    def set_delta(self, on):
+        """Delta-f storage (vpfp_plan_set_delta): f arrays hold f - M(v)."""
+        check(library().vpfp_plan_set_delta(self.handle, 1 if on else 0))
+
+    def background(self, f, sign, stream=None):
+        """f += sign * M(v) in place on a device tensor of the plan's storage type
+        (sign -1: f to the stored deviation, +1 back)."""
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream()
+        check(library().vpfp_plan_background(self.handle, _c_p(f.data_ptr()), int(sign), _c_p(s.cuda_stream)))
 
     def kernel_times(self):
         """{kind: (total_ms, launches)} for kinds row, drift, field, other."""

@@ -332,6 +332,9 @@ struct vpfp_plan_s {
   int xcd = 1;     // drift tile order (xcd_block); never slower in the measured shapes
   int dtiles = 1;  // column tiles per drift workgroup
   double tuned_us = 0.0;
+  int delta = 0;                 // f stored as f - M(v)
+  double* bg = nullptr;          // M(v_m) [nv] (FP64) and its DFT [nv] (storage precision)
+  void* bg_hat = nullptr;
   double *kx = nullptr, *okx = nullptr, *v = nullptr, *kv = nullptr;
   void* tw_v = nullptr;          // cx<T> [lo | hi] tables for nv
   void* tw_x = nullptr;          // cx<T> [lo | hi] tables for nx
@@ -494,6 +497,10 @@ vp::RowArgs<T> row_args(vpfp_plan_t p, const void* fin, void* fout) {
   a.tw = reinterpret_cast<const cx<T>*>(p->tw_v);
   a.nu_dt = p->nu * p->dt;
   a.op_dg = p->op == VPFP_OP_DG;
+  if (p->delta) {
+    a.bg = p->bg;
+    a.bg_hat = reinterpret_cast<const cx<T>*>(p->bg_hat);
+  }
   return a;
 }
 
@@ -977,6 +984,85 @@ int check_tuning(const vpfp_tuning* t, int nv) {
 }  // namespace
 
 // ====================================================================== C ABI
+namespace {
+
+// unit Maxwellian on a v grid (v_m = v0 + m dvv) and its DFT
+// sum_m M_m exp(-2 pi i k m / N) (direct sum in FP64 with an exact twiddle index)
+void background_tables(int N, double v0, double dvv, std::vector<double>& m, std::vector<cx<double>>& hat) {
+  m.assign(N, 0.0);
+  for (int i = 0; i < N; ++i) {
+    const double v = v0 + i * dvv;
+    m[i] = std::exp(-0.5 * v * v) / std::sqrt(6.283185307179586);
+  }
+  std::vector<double> wr(N), wi(N);
+  for (int j = 0; j < N; ++j) {
+    wr[j] = std::cos(6.283185307179586 * j / N);
+    wi[j] = -std::sin(6.283185307179586 * j / N);
+  }
+  hat.assign(N, vp::mk<double>(0.0, 0.0));
+  for (int k = 0; k < N; ++k) {
+    double re = 0.0, im = 0.0;
+    for (int i = 0; i < N; ++i) {
+      const int j = (int)(((long long)k * i) % N);
+      re += m[i] * wr[j];
+      im += m[i] * wi[j];
+    }
+    hat[k] = vp::mk<double>(re, im);
+  }
+}
+
+std::vector<cx<float>> to_float(const std::vector<cx<double>>& h) {
+  std::vector<cx<float>> out(h.size());
+  for (size_t k = 0; k < h.size(); ++k) out[k] = vp::mk<float>((float)h[k].re, (float)h[k].im);
+  return out;
+}
+
+// once per batch
+int batch_background(vpfp_batch_s* b) {
+  if (b->bg) return VPFP_OK;
+  const int N = b->nv;
+  std::vector<double> m;
+  std::vector<cx<double>> hat;
+  background_tables(N, b->v0, b->dvv, m, hat);
+  if (int rc = bput(b, &b->bg, m)) return rc;
+  if (b->prec == VPFP_F64) {
+    cx<double>* d = nullptr;
+    if (int rc = bput(b, &d, hat)) return rc;
+    b->bg_hat = d;
+  } else {
+    cx<float>* d = nullptr;
+    if (int rc = bput(b, &d, to_float(hat))) return rc;
+    b->bg_hat = d;
+  }
+  return VPFP_OK;
+}
+
+// once per plan
+int plan_background(vpfp_plan_s* p) {
+  if (p->bg) return VPFP_OK;
+  std::vector<double> m;
+  std::vector<cx<double>> hat;
+  background_tables(p->nv, p->v0, p->dvv, m, hat);
+  if (int rc = put(p, &p->bg, m)) return rc;
+  if (p->prec == VPFP_F64) {
+    cx<double>* d = nullptr;
+    if (int rc = put(p, &d, hat)) return rc;
+    p->bg_hat = d;
+  } else {
+    cx<float>* d = nullptr;
+    if (int rc = put(p, &d, to_float(hat))) return rc;
+    p->bg_hat = d;
+  }
+  return VPFP_OK;
+}
+
+int no_delta(vpfp_plan_s* p, const char* what) {
+  if (p->delta) return fail(VPFP_EUNSUPPORTED, std::string(what) + " reads f itself: not with delta-f storage");
+  return VPFP_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int vpfp_version(void) { return 1; }
@@ -1050,6 +1136,33 @@ int vpfp_plan_destroy(vpfp_plan_t p) {
   return VPFP_OK;
 }
 
+int vpfp_plan_set_delta(vpfp_plan_t p, int on) {
+  if (int rc = check_plan(p)) return rc;
+  if (on != 0 && on != 1) return fail(VPFP_EBADARG, "delta must be 0 or 1");
+  if (on && !p->nv_pow2) return fail(VPFP_EUNSUPPORTED, "delta-f storage needs a power-of-two nv");
+  if (on)
+    if (int rc = plan_background(p)) return rc;
+  p->delta = on;
+  return VPFP_OK;
+}
+
+int vpfp_plan_background(vpfp_plan_t p, void* f, int sign, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!f) return fail(VPFP_EBADARG, "null array");
+  if (sign != 1 && sign != -1) return fail(VPFP_EBADARG, "sign must be 1 or -1");
+  if (int rc = plan_background(p)) return rc;
+  const size_t n = (size_t)p->nx * p->nv;
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  if (p->prec == VPFP_F64)
+    hipLaunchKernelGGL(k_background<double>, dim3(grid), dim3(256), 0, S(stream), reinterpret_cast<double*>(f), n,
+                       p->nv, p->bg, (double)sign);
+  else
+    hipLaunchKernelGGL(k_background<float>, dim3(grid), dim3(256), 0, S(stream), reinterpret_cast<float*>(f), n,
+                       p->nv, p->bg, (double)sign);
+  VPFP_HIP(hipGetLastError());
+  return VPFP_OK;
+}
+
 int vpfp_vdfdx(vpfp_plan_t p, const void* fin, void* fout, double dt, void* stream) {
   if (int rc = check_plan(p)) return rc;
   if (int rc = need_spectral_x(p)) return rc;
@@ -1068,6 +1181,7 @@ int vpfp_edfdv(vpfp_plan_t p, const void* fin, void* fout, const double* e, doub
 
 int vpfp_compute_charges(vpfp_plan_t p, const void* f, double* n_out, void* stream) {
   if (int rc = check_plan(p)) return rc;
+  if (int rc = no_delta(p, "vpfp_compute_charges")) return rc;
   if (!f || !n_out) return fail(VPFP_EBADARG, "null array");
   return p->prec == VPFP_F64 ? do_density<double>(p, f, p->dv, n_out, S(stream))
                              : do_density<float>(p, f, p->dv, n_out, S(stream));
@@ -1095,6 +1209,7 @@ int vpfp_field(vpfp_plan_t p, const void* f, const double* driver, double* e_out
 
 int vpfp_collision_diagonals(vpfp_plan_t p, const void* f, double* a, double* b, double* c, void* stream) {
   if (int rc = check_plan(p)) return rc;
+  if (int rc = no_delta(p, "vpfp_collision_diagonals")) return rc;
   if (!f || !a || !b || !c) return fail(VPFP_EBADARG, "null array");
   return p->prec == VPFP_F64 ? do_diag<double>(p, f, a, b, c, S(stream)) : do_diag<float>(p, f, a, b, c, S(stream));
 }
@@ -1102,6 +1217,7 @@ int vpfp_collision_diagonals(vpfp_plan_t p, const void* f, double* a, double* b,
 int vpfp_tridiag_solve(vpfp_plan_t p, const double* a, const double* b, const double* c, const void* d, void* x,
                        void* stream) {
   if (int rc = check_plan(p)) return rc;
+  if (int rc = no_delta(p, "vpfp_tridiag_solve")) return rc;
   if (int rc = need_spectral_v(p)) return rc;
   if (!a || !b || !c || !d || !x) return fail(VPFP_EBADARG, "null array");
   return p->prec == VPFP_F64 ? do_tri<double>(p, a, b, c, d, x, S(stream))
@@ -1140,6 +1256,7 @@ int vpfp_moments(vpfp_plan_t p, const void* f, double* moments, void* stream) {
 
 int vpfp_x_modes(vpfp_plan_t p, const void* f, int n_modes, double* out, void* stream) {
   if (int rc = check_plan(p)) return rc;
+  if (int rc = no_delta(p, "vpfp_x_modes")) return rc;
   if (!f || !out) return fail(VPFP_EBADARG, "null array");
   if (n_modes < 1 || n_modes > p->nx) return fail(VPFP_EBADARG, "n_modes must be in [1, nx]");
   const int total = n_modes * p->nv;
@@ -1223,49 +1340,6 @@ int vpfp_batch_set_closure(vpfp_batch_t b, vpfc_net_t net) {
   return VPFP_OK;
 }
 
-namespace {
-
-// unit Maxwellian on the batch's v grid and its DFT sum_m M_m exp(-2 pi i k m / N)
-// (direct sum in FP64 with an exact twiddle index, once per batch)
-int batch_background(vpfp_batch_s* b) {
-  if (b->bg) return VPFP_OK;
-  const int N = b->nv;
-  std::vector<double> m(N);
-  for (int i = 0; i < N; ++i) {
-    const double v = b->v0 + i * b->dvv;
-    m[i] = std::exp(-0.5 * v * v) / std::sqrt(6.283185307179586);
-  }
-  std::vector<double> wr(N), wi(N);
-  for (int j = 0; j < N; ++j) {
-    wr[j] = std::cos(6.283185307179586 * j / N);
-    wi[j] = -std::sin(6.283185307179586 * j / N);
-  }
-  std::vector<cx<double>> hat(N);
-  for (int k = 0; k < N; ++k) {
-    double re = 0.0, im = 0.0;
-    for (int i = 0; i < N; ++i) {
-      const int j = (int)(((long long)k * i) % N);
-      re += m[i] * wr[j];
-      im += m[i] * wi[j];
-    }
-    hat[k] = vp::mk<double>(re, im);
-  }
-  if (int rc = bput(b, &b->bg, m)) return rc;
-  if (b->prec == VPFP_F64) {
-    cx<double>* d = nullptr;
-    if (int rc = bput(b, &d, hat)) return rc;
-    b->bg_hat = d;
-  } else {
-    std::vector<cx<float>> hf(N);
-    for (int k = 0; k < N; ++k) hf[k] = vp::mk<float>((float)hat[k].re, (float)hat[k].im);
-    cx<float>* d = nullptr;
-    if (int rc = bput(b, &d, hf)) return rc;
-    b->bg_hat = d;
-  }
-  return VPFP_OK;
-}
-
-}  // namespace
 
 int vpfp_batch_set_delta(vpfp_batch_t b, int on) {
   if (!b) return fail(VPFP_EBADARG, "null batch");
