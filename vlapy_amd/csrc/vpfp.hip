@@ -168,8 +168,13 @@ int launch_row_q(const vp::RowArgs<T>& a, hipStream_t s) {
 // 16 <= M <= 2048, 16 for M >= 256), 0 = the row-count rule above
 template <class T, int M>
 int launch_row(const vp::RowArgs<T>& a, hipStream_t s, int q) {
-  // delta-f storage: one launch shape
-  if (a.bg_hat) return launch_row_q<T, M, qfor(M), true>(a, s);
+  // delta-f storage: Q = 4 or 8 (q = 16 runs at 8)
+  if (a.bg_hat) {
+    if constexpr (M >= 16 && M <= 2048) {
+      if (q == 4 || (q == 0 && (long long)a.nx * (M / 8) < 16384)) return launch_row_q<T, M, 4, true>(a, s);
+    }
+    return launch_row_q<T, M, qfor(M), true>(a, s);
+  }
   if constexpr (M >= 256 && M <= 4096) {
     if (q == 16) return launch_row_q<T, M, 16>(a, s);
   }
