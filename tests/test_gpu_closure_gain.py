@@ -1,6 +1,6 @@
 """What the trained closure (vlapy_amd/data/closure_c5.npz, written by
 `python -m vlapy_amd.train_closure --relative 0 --conserve 1`) does against the
-method without it, judged by the fine grid.
+method without it, measured against the fine grid.
 
 32 held-out runs (draw_instances(32, seed=12): k ~ U[0.25, 0.45], amplitude
 10^U[-4, -1], nu 10^U[-5, -2]; nx = 64, dt = 0.05, 4000 steps) on the fine

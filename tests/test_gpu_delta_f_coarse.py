@@ -1,5 +1,5 @@
 """FP32 delta-f storage on the coarse grid of the closure configuration,
-judged by the fine grid.
+measured against the fine grid.
 
 32 held-out runs (draw_instances(32, seed=12): k ~ U[0.25, 0.45], amplitude
 10^U[-4, -1], nu 10^U[-5, -2]; nx = 64, dt = 0.05, 4000 steps) on the fine

@@ -6,7 +6,7 @@ step:  f(x, .) <- f(x, .) + out_scale * net(in_scale * f(x, .)).
 Training, inference and the Adam update run in the BF16/MFMA kernels of
 libvpfp.so (include/vpfc.h); this module holds the host side: parameter
 initialisation and files, training data from the method's own fine-grid runs,
-the training loop and the rollouts that judge the closure against the fine
+the training loop and the rollouts that evaluate the closure against the fine
 grid.
 """
 
@@ -177,7 +177,7 @@ class Closure:
 
 
 # ---------------------------------------------------------------------------
-# data, training and judgement
+# data, training and evaluation
 # ---------------------------------------------------------------------------
 NX, DT, VMAX, NV_FINE = 64, 0.05, 6.4, 4096
 

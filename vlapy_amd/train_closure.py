@@ -1,12 +1,12 @@
 """Train the coarse-velocity closure on the method's own fine-grid runs and
-judge it against them.
+evaluate it against them.
 
     python -m vlapy_amd.train_closure --out vlapy_amd/data/closure_c5.npz
 
 Training runs: --instances draws (vlapy_amd.closure.draw_instances, seed
 --seed) at nx = 64, dt = 0.05 for --steps steps on the fine grid (nv = 4096,
 FP64); every --every steps the state is averaged onto nv = 256 and paired
-with the per-step correction over --span steps.  Judgement: --eval-instances
+with the per-step correction over --span steps.  Evaluation: --eval-instances
 fresh draws (seed --eval-seed) run on the fine grid (FP64), on the coarse
 grid (FP32) and on the coarse grid with the closure; the field-energy
 history error and the damping-rate error of each coarse run against the fine
@@ -23,7 +23,7 @@ import torch
 from . import closure as CL
 
 
-def judge(net, inst, steps):
+def evaluate(net, inst, steps):
     t = CL.DT * (np.arange(steps) + 1)
     out = {}
     t0 = time.perf_counter()
@@ -76,7 +76,7 @@ def main():
     torch.cuda.synchronize()
     train_s = time.perf_counter() - t0
     del x, t
-    res = judge(net, CL.draw_instances(a.eval_instances, a.eval_seed), a.steps)
+    res = evaluate(net, CL.draw_instances(a.eval_instances, a.eval_seed), a.steps)
     res.update({"train_seconds": train_s, "final_loss": hist[-1][1], "config": vars(a)})
     net.save(a.out, metrics=json.dumps(res))
     print(json.dumps(res), flush=True)
