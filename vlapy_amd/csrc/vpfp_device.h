@@ -318,27 +318,36 @@ VP_HD cx<T> cis_neg_m1(double th) {
 
 // PhaseMul on a stored deviation f - B from a fixed background B whose
 // unnormalised spectrum is bhat [N] (storage precision): the shift of B itself
-// enters as F'_k += bhat_k (P_k - 1), F'_{k+M} += bhat_{k+M} (conj P_{M-k} - 1),
-// with c_k = P_k - 1 carried by c_{k+s} = c_k + c_s + c_k c_s (no cancellation
-// for small shifts).  couple() calls init() once, then bg_add(k, ...) right
-// after pair(k, ...); the c state is formed in init() so that it is not live
-// across the forward transform.
+// enters as F'_k += bhat_k (P_k - 1), F'_{k+M} += bhat_{k+M} (conj P_{M-k} - 1).
+// c_k = P_k - 1 is carried by c_{k+s} = c_k + c_s + c_k c_s (no cancellation
+// for small shifts) and the phases themselves are P_k = 1 + c_k,
+// P_M = conj(1 + cpm) with cpm = exp(+i M phi) - 1.  couple() calls init()
+// once, then pair(k, ...) and bg_add(k, ...) for each k in turn; all of the
+// state is formed in init(), after the forward transform.
 template <class T>
 struct BgPhaseMul {
   static constexpr bool kBg = true;
-  PhaseMul<T> ph;
   double phi;
   int M, t, TPT;
   const cx<T>* bhat;
   cx<T> cm, cs, cpm;  // P_k - 1, P_TPT - 1, exp(+i M phi) - 1
   VP_HD BgPhaseMul(double phi_, int M_, int t_, int TPT_, const cx<T>* bhat_)
-      : ph(phi_, M_, t_, TPT_), phi(phi_), M(M_), t(t_), TPT(TPT_), bhat(bhat_) {}
+      : phi(phi_), M(M_), t(t_), TPT(TPT_), bhat(bhat_) {}
   VP_HD void init() {
     cm = cis_neg_m1<T>((double)t * phi);
     cs = cis_neg_m1<T>((double)TPT * phi);
     cpm = cis_neg_m1<T>(-(double)M * phi);
   }
-  VP_HD void pair(int k, cx<T>& pk, cx<T>& pmk) { ph.pair(k, pk, pmk); }
+  VP_HD void pair(int k, cx<T>& pk, cx<T>& pmk) {
+    const cx<T> pm = mk<T>(T(1) + cpm.re, -cpm.im);
+    if (k == 0) {
+      pk = mk<T>(T(1), T(0));
+      pmk = mk<T>(pm.re, T(0));
+    } else {
+      pk = mk<T>(T(1) + cm.re, cm.im);
+      pmk = mulc(pm, pk);
+    }
+  }
   VP_HD void bg_add(int k, cx<T>& G1, cx<T>& G2) {
     const cx<T> B1 = bhat[k], B2 = bhat[k + M];
     if (k == 0) {
