@@ -222,8 +222,8 @@ SCAN_GRAPH_STEPS = 100
 
 
 def scan_window(p, i, e2, t):
-    """Fit window for instance i: after one period, before trapping
-    (2 pi / sqrt(amp)), half the recurrence time 2 pi / (k dv) and the
+    """Fit window for instance i: after one period, before half the trapping
+    time (pi / sqrt(amp)), half the recurrence time 2 pi / (k dv) and the
     point where <E^2> has fallen 1e8 below its start (FP32 noise)."""
     import numpy as np
 

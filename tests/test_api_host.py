@@ -140,3 +140,20 @@ def test_bench_refuses_several_gpus():
                          text=True, timeout=300)
     assert res.returncode == 2
     assert "one GPU" in res.stderr
+
+
+def test_plan_background_validates_f_before_the_device_call():
+    """Plan.background writes nx * nv elements of the plan's storage type:
+    a wrong shape, dtype or layout, or a host tensor, raises ValueError
+    before any C call (the handle here is never used)."""
+    import types
+
+    import torch
+
+    from vlapy_amd._lib import Plan
+
+    fake = types.SimpleNamespace(nx=4, nv=8, precision="fp64", handle=None)
+    for bad in (torch.zeros(4, 8, dtype=torch.float32), torch.zeros(4, 16, dtype=torch.float64),
+                torch.zeros(8, 4, dtype=torch.float64).t(), torch.zeros(4, 8, dtype=torch.float64)):
+        with pytest.raises(ValueError):
+            Plan.background(fake, bad, 1)

@@ -84,3 +84,39 @@ def test_adam_first_step_and_quadratic():
     for t in range(1, 3001):
         p, m, v = C.adam(p, m, v, a * p, t, lr=1e-2)
     assert np.abs(p).max() < 1e-2
+
+
+@pytest.mark.parametrize("with_bg", [False, True])
+def test_forward_equals_torch_conv1d(with_bg):
+    """The declared network, evaluated by torch.nn.functional.conv1d (zero
+    padding 2, cross-correlation) in float64, equals the oracle's exact-mode
+    forward pass: an implementation that shares no code with the oracle."""
+    import torch.nn.functional as F
+
+    p, f, _ = _problem(rows=3, seed=5)
+    v = np.linspace(-6.4 + 0.025, 6.4 - 0.025, 256)
+    bg = np.exp(-v ** 2 / 2) / np.sqrt(2 * np.pi) if with_bg else None
+    y, _ = C.forward(p, f, 2.5, bf16=False, bg=bg)
+
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64))  # noqa: E731
+    o = C.OFF
+    W1 = t(p[o["W1"][0]:o["W1"][1]].reshape(64, 1, 5))
+    # oracle layout [co][k][ci] -> torch [co][ci][k]
+    W2 = t(p[o["W2"][0]:o["W2"][1]].reshape(64, 5, 64).transpose(0, 2, 1))
+    W3 = t(p[o["W3"][0]:o["W3"][1]].reshape(64, 5, 64).transpose(0, 2, 1))
+    W4 = t(p[o["W4"][0]:o["W4"][1]].reshape(5, 64).T[None, :, :])
+    b = {n: t(p[o[n][0]:o[n][1]]) for n in ("b1", "b2", "b3", "b4")}
+    ff = np.asarray(f, dtype=np.float64)
+    if with_bg:
+        d = ff - bg[None, :]
+        s = np.sqrt(np.mean(d * d, axis=1, keepdims=True) + 1e-30)
+        x = 2.5 * d / s
+    else:
+        x = 2.5 * ff
+    h = t(x)[:, None, :]
+    h = torch.tanh(F.conv1d(h, W1, b["b1"], padding=2))
+    h = torch.tanh(F.conv1d(h, W2, b["b2"], padding=2))
+    h = torch.tanh(F.conv1d(h, W3, b["b3"], padding=2))
+    ref = F.conv1d(h, W4, b["b4"], padding=2)[:, 0, :].numpy()
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    assert err <= 1e-13, err

@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 NX, NV, K, AMP, NU, DT, STEPS = 32, 256, 0.3, 0.05, 1e-3, 0.05, 200
 
 
-def _run(precision, delta):
+def _run(precision, delta, row_q=None):
     import torch
 
     from vlapy_amd import ops
@@ -27,6 +27,8 @@ def _run(precision, delta):
 
     x, kx, okx, v, kv, dv = instance_grids(NX, K, NV, 6.4)
     plan = Plan(NX, NV, precision, kx, okx, v, kv, dv, NU, DT, "dg")
+    if row_q is not None:
+        plan.set_tuning(row_q=row_q)
     dev = torch.device("cuda:0")
     m = np.exp(-v ** 2 / 2.0) / np.sqrt(2.0 * np.pi)
     f0 = (1.0 + AMP * np.cos(K * x))[:, None] * m[None, :]
@@ -48,9 +50,12 @@ def _rel(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
-def test_plan_delta_f_fp64_reproduces_plain_plan():
-    _, eh, mo, f = _run("fp64", False)
-    _, ehd, mod, fd = _run("fp64", True)
+@pytest.mark.parametrize("row_q", [0, 4, 8, 16])
+def test_plan_delta_f_fp64_reproduces_plain_plan(row_q):
+    """Every row-kernel shape: 0 (row-count rule), 4, 8, and 16 (the Q = 8
+    kernel under delta-f storage; Q = 16 does not exist at nv = 256)."""
+    _, eh, mo, f = _run("fp64", False, row_q)
+    _, ehd, mod, fd = _run("fp64", True, row_q)
     print("fp64 delta-f vs plain: E %.2e moments %.2e f %.2e" % (_rel(ehd, eh), _rel(mod[:, :6], mo[:, :6]),
                                                                 _rel(fd, f)))
     assert _rel(ehd, eh) < 1e-11

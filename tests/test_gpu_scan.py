@@ -106,17 +106,26 @@ def test_scan_lenard_bernstein_matches_oracle():
         assert err < 1e-11, (i, err)
 
 
+@pytest.mark.parametrize("row_q", [0, 4, 8, 16])
 @pytest.mark.parametrize("precision,tol_e2,tol_f", [("fp64", 1e-11, 1e-12), ("fp32", 2e-3, 2e-5)])
-def test_scan_delta_f_matches_oracle(precision, tol_e2, tol_f):
+def test_scan_delta_f_matches_oracle(precision, tol_e2, tol_f, row_q):
     """Delta-f storage (f - M stored, vpfp_batch_set_delta) runs the same
     algorithm: FP64 within the plain scan's 1e-11 of the oracle, f (the
     deviation plus the background, vpfp_batch_background) within 1e-12; FP32
-    within the plain FP32 scan's bounds."""
+    within the plain FP32 scan's bounds.  Every row-kernel shape of the
+    delta-f rows: row_q 0 (the row-count rule, Q = 4 at these sizes), 4, 8
+    and 16 (which runs the Q = 8 kernel under delta-f storage)."""
     from vlapy_amd.scan import Scan
 
     p = _params()
     n, dt = 20, 0.1
     sc = Scan(p, dt=dt, op="dg", precision=precision, delta=True)
+    for c in sc.classes:
+        t = c.tuning()
+        t["row_q"] = row_q
+        if row_q == 4 and c.nv > 4096:
+            continue
+        c.set_tuning(t)
     e2 = sc.run(n)
     for i in range(p["k"].size):
         f_o, e2_o = _oracle(p, i, n, dt)

@@ -204,8 +204,15 @@ class Plan:
         (sign -1: f to the stored deviation, +1 back)."""
         import torch
 
+        from . import ops
+
+        # (nx, nv), the storage dtype, contiguous, on the device: the kernel
+        # writes nx * nv elements of that type
+        ptr = ops._f(self, f)
+        if int(sign) not in (1, -1):
+            raise ValueError("sign must be 1 or -1")
         s = stream if stream is not None else torch.cuda.current_stream()
-        check(library().vpfp_plan_background(self.handle, _c_p(f.data_ptr()), int(sign), _c_p(s.cuda_stream)))
+        check(library().vpfp_plan_background(self.handle, _c_p(ptr), int(sign), _c_p(s.cuda_stream)))
 
     def kernel_times(self):
         """{kind: (total_ms, launches)} for kinds row, drift, field, other."""

@@ -348,7 +348,7 @@ def rollout(inst, nv, precision, n_steps, net=None, delta=False):
 
 def damping_errors(inst, e2_ref, e2, t):
     """|gamma / gamma_ref - 1| per instance from the standing-wave estimator on
-    a window after one period, before trapping (2 pi / sqrt(amp)), before
+    a window after one period, before half the trapping time (pi / sqrt(amp)), before
     coarse-grid recurrence (pi / (k dv)) and before <E^2> falls 1e8 below its
     start; NaN where the window holds fewer than three maxima."""
     from .diagnostics.low_level_helpers import get_standing_wave_rate_and_frequency as rate

@@ -27,7 +27,7 @@
 
 namespace {
 
-std::string g_err;
+thread_local std::string g_err;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;

@@ -1,5 +1,10 @@
 // HIP kernels and the C ABI (include/vpfp.h) of the VPFP step on gfx950.
 //
+// This file holds the field, density, diagonal, x-mode, background and
+// tridiagonal kernels, the plan and batch objects, the autotuner and the C
+// ABI; the row kernels are in vpfp_row.hip and the drift kernels in
+// vpfp_drift.hip (one object per storage type, compiled in parallel).
+//
 // Launch shapes (wave64):
 //   row kernel   M = nv/2 complex points per row, Q = min(8, M) per thread,
 //                TPT = M/Q threads per row, max(1, 64/TPT) rows per workgroup
@@ -14,20 +19,36 @@
 #include <string>
 #include <vector>
 
-#include "vpfp.h"
 #include "vpfc.h"
-#include "vpfp_kernels.h"
+#include "vpfp.h"
+#include "vpfp_launch.h"
 
 using vp::cx;
 
-namespace {
+namespace vpfp_impl {
 
+namespace {
 thread_local std::string g_err;
+}
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
 }
+
+int launched() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(VPFP_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return VPFP_OK;
+}
+
+const char* last_error() { return g_err.c_str(); }
+
+}  // namespace vpfp_impl
+
+using namespace vpfp_impl;
+
+namespace {
 
 #define VPFP_HIP(x)                                                                 \
   do {                                                                              \
@@ -35,36 +56,7 @@ int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess) return fail(VPFP_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-struct GpuCtx {
-  int tid, nthr, bid, nblk, bidy;
-  __device__ __forceinline__ void sync() { __syncthreads(); }
-  __device__ __forceinline__ void fence() { __threadfence(); }
-  __device__ __forceinline__ unsigned arrive(unsigned* c) { return atomicAdd(c, 1u); }
-  __device__ __forceinline__ void reset(unsigned* c) { __hip_atomic_store(c, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-  __device__ __forceinline__ double load_coherent(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __device__ __forceinline__ double shfl_xor(double v, int m, int w) { return __shfl_xor(v, m, w); }
-  __device__ __forceinline__ float shfl_xor(float v, int m, int w) { return __shfl_xor(v, m, w); }
-};
-
-#define VPFP_CTX GpuCtx c{(int)threadIdx.x, (int)blockDim.x, (int)blockIdx.x, (int)gridDim.x, (int)blockIdx.y}
-
 // ------------------------------------------------------------------ kernels
-template <class T, int M, int Q, bool BG = false>
-__global__ __launch_bounds__((vp::RowCfg<T, M, Q>::wg), (vp::RowCfg<T, M, Q>::min_waves)) void k_row(vp::RowArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char lds[];
-  VPFP_CTX;
-  vp::row_body<T, M, Q, GpuCtx, BG>(c, lds, a);
-}
-
-template <class T, int M, int Q, int C>
-__global__ __launch_bounds__((vp::DriftCfg<T, M, Q, C>::wg)) void k_drift(vp::DriftArgs<T> a) {
-  extern __shared__ __align__(16) unsigned char lds[];
-  VPFP_CTX;
-  vp::drift_body<T, M, Q, C>(c, lds, a);
-}
-
 constexpr int kFT = 256;
 
 // grid (nblk, 1): one field solve over nblk workgroups; grid (1, n): the n
@@ -119,26 +111,6 @@ __global__ __launch_bounds__((vp::RowCfg<T, M, Q>::wg)) void k_tri(vp::TriArgs<T
 }
 
 // ------------------------------------------------------------------ launch helpers
-int launched() {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(VPFP_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
-  return VPFP_OK;
-}
-
-template <class K>
-int set_lds(K kern, int bytes) {
-  if (bytes > 65536) {
-    hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-    if (e != hipSuccess) return fail(VPFP_EHIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-  }
-  return VPFP_OK;
-}
-
-constexpr int qfor(int M) { return M < 8 ? M : 8; }
-
-// LDS per workgroup on gfx950
-constexpr int kMaxLds = 160 * 1024;
-
 // drift workgroups: one per `per` column tiles (at most 1024 for a single
 // grid, which then strides); each workgroup writes one partial density row
 constexpr int kDriftMaxWG = 1024;
@@ -151,39 +123,6 @@ int drift_wgs(int nv, int C, int per, bool batch) {
 // field workgroups: one per 64 x values (the last to finish solves Poisson)
 int field_grid(int nx) { return nx >= 128 ? nx / 64 : 1; }
 
-template <class T, int M, int Q, bool BG = false>
-int launch_row_q(const vp::RowArgs<T>& a, hipStream_t s) {
-  using Cfg = vp::RowCfg<T, M, Q>;
-  auto kern = k_row<T, M, Q, BG>;
-  if (int rc = set_lds(kern, Cfg::lds)) return rc;
-  const int grid = (a.nx + Cfg::rows - 1) / Cfg::rows;
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(Cfg::wg), Cfg::lds, s, a);
-  return launched();
-}
-
-// Q = 8 values per thread fills the GPU when there are many rows; with few
-// rows (fewer than ~16k threads at Q = 8) Q = 4 spreads each row over twice
-// as many threads and shortens the per-row latency chain.
-// q: 4, 8 or 16 forces that many values per thread (where it exists: 4 for
-// 16 <= M <= 2048, 16 for M >= 256), 0 = the row-count rule above
-template <class T, int M>
-int launch_row(const vp::RowArgs<T>& a, hipStream_t s, int q) {
-  // delta-f storage: Q = 4 or 8 (q = 16 runs at 8)
-  if (a.bg_hat) {
-    if constexpr (M >= 16 && M <= 2048) {
-      if (q == 4 || (q == 0 && (long long)a.nx * (M / 8) < 16384)) return launch_row_q<T, M, 4, true>(a, s);
-    }
-    return launch_row_q<T, M, qfor(M), true>(a, s);
-  }
-  if constexpr (M >= 256 && M <= 4096) {
-    if (q == 16) return launch_row_q<T, M, 16>(a, s);
-  }
-  if constexpr (M >= 16 && M <= 2048) {
-    if (q == 4 || (q == 0 && (long long)a.nx * (M / 8) < 16384)) return launch_row_q<T, M, 4>(a, s);
-  }
-  return launch_row_q<T, M, qfor(M)>(a, s);
-}
-
 template <class T, int M>
 int launch_tri(const vp::TriArgs<T>& a, hipStream_t s) {
   constexpr int Q = qfor(M);
@@ -193,33 +132,6 @@ int launch_tri(const vp::TriArgs<T>& a, hipStream_t s) {
   const int grid = (a.nx + Cfg::rows - 1) / Cfg::rows;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(Cfg::wg), Cfg::lds, s, a);
   return launched();
-}
-
-template <class T, int M, int C>
-int launch_drift_c(const vp::DriftArgs<T>& a, hipStream_t s, int gx, int gy) {
-  constexpr int Q = M <= 64 ? (M < 4 ? M : 4) : 8;
-  using Cfg = vp::DriftCfg<T, M, Q, C>;
-  auto kern = k_drift<T, M, Q, C>;
-  if (Cfg::wg > 1024 || Cfg::lds > kMaxLds)
-    return fail(VPFP_EUNSUPPORTED, "drift tile of " + std::to_string(C) + " columns does not fit a workgroup");
-  if (a.fuse_field && !Cfg::tail) return fail(VPFP_EUNSUPPORTED, "field tail not available at this size");
-  if (int rc = set_lds(kern, Cfg::lds)) return rc;
-  // gx workgroups stride over the nv / C column tiles (drift_wgs); gy > 0:
-  // a batch of gy instances (grid y = instance)
-  const dim3 grid = gy > 0 ? dim3(gx, gy) : dim3(gx);
-  hipLaunchKernelGGL(kern, grid, dim3(Cfg::wg), Cfg::lds, s, a);
-  return launched();
-}
-
-template <class T, int M>
-int launch_drift(const vp::DriftArgs<T>& a, int C, hipStream_t s, int gx, int gy) {
-  switch (C) {
-    case 4: return launch_drift_c<T, M, 4>(a, s, gx, gy);
-    case 8: return launch_drift_c<T, M, 8>(a, s, gx, gy);
-    case 16: return launch_drift_c<T, M, 16>(a, s, gx, gy);
-    case 32: return launch_drift_c<T, M, 32>(a, s, gx, gy);
-  }
-  return fail(VPFP_EUNSUPPORTED, "drift tile width");
 }
 
 template <int M>
@@ -233,32 +145,9 @@ int launch_field(const vp::FieldArgs& a, hipStream_t s, int n_inst) {
   return launched();
 }
 
-#define VPFP_SWITCH_M(M, CALL, MAXM)                 \
-  switch (M) {                                       \
-    case 2: return CALL(2);                          \
-    case 4: return CALL(4);                          \
-    case 8: return CALL(8);                          \
-    case 16: return CALL(16);                        \
-    case 32: return CALL(32);                        \
-    case 64: return CALL(64);                        \
-    case 128: return CALL(128);                      \
-    case 256: return CALL(256);                      \
-    case 512: return CALL(512);                      \
-    case 1024: if (MAXM >= 1024) return CALL(1024);  \
-      break;                                         \
-    case 2048: if (MAXM >= 2048) return CALL(2048);  \
-      break;                                         \
-    case 4096: if (MAXM >= 4096) return CALL(4096);  \
-      break;                                         \
-  }
-
 template <class T>
 int row_dispatch(const vp::RowArgs<T>& a, hipStream_t s, int q = 0) {
-  const int M = a.nv / 2;
-#define CALL(m) launch_row<T, ((m) <= 4096 ? (m) : 2)>(a, s, q)
-  VPFP_SWITCH_M(M, CALL, 4096)
-#undef CALL
-  return fail(VPFP_EUNSUPPORTED, "nv must be a power of two in [4, 8192]");
+  return a.bg_hat ? row_dispatch_bg(a, s, q) : row_dispatch_plain(a, s, q);
 }
 
 template <class T>
@@ -272,11 +161,7 @@ int tri_dispatch(const vp::TriArgs<T>& a, hipStream_t s) {
 
 template <class T>
 int drift_dispatch(const vp::DriftArgs<T>& a, int C, int gx, hipStream_t s, int gy = 0) {
-  const int M = a.nx / 2;
-#define CALL(m) launch_drift<T, ((m) <= 1024 ? (m) : 2)>(a, C, s, gx, gy)
-  VPFP_SWITCH_M(M, CALL, 1024)
-#undef CALL
-  return fail(VPFP_EUNSUPPORTED, "nx must be a power of two in [4, 2048] for the x-advection");
+  return vpfp_impl::drift_dispatch(a, C, gx, s, gy);
 }
 
 // n_inst > 0: a batch of n_inst instance-major field solves (pow2 only)
@@ -326,6 +211,34 @@ bool uniform_grid(const double* v, int n, double* v0, double* dvv) {
 
 }  // namespace
 
+// Streams an object enqueued work on.  Destroying the object waits for
+// them (its buffers must outlive that work); after more than kMax distinct
+// streams it waits for the whole device instead.  The streams must still
+// exist when the object is destroyed.
+struct StreamLog {
+  static constexpr int kMax = 8;
+  hipStream_t s[kMax] = {};
+  int n = 0;
+  bool overflow = false;
+  hipStream_t use(void* st) {
+    hipStream_t h = reinterpret_cast<hipStream_t>(st);
+    for (int i = 0; i < n; ++i)
+      if (s[i] == h) return h;
+    if (n < kMax)
+      s[n++] = h;
+    else
+      overflow = true;
+    return h;
+  }
+  void drain() {
+    if (overflow) {
+      (void)hipDeviceSynchronize();
+      return;
+    }
+    for (int i = 0; i < n; ++i) (void)hipStreamSynchronize(s[i]);
+  }
+};
+
 struct vpfp_plan_s {
   int nx = 0, nv = 0, prec = VPFP_F64, op = VPFP_OP_LB;
   double dv = 0, nu = 0, dt = 0;
@@ -357,6 +270,7 @@ struct vpfp_plan_s {
   std::vector<hipEvent_t> ev;          // pool, 2 per timed launch
   std::vector<int> ev_kind;
   size_t ev_used = 0;
+  StreamLog streams;
   std::vector<void*> allocs;
   ~vpfp_plan_s() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
@@ -442,8 +356,6 @@ int drift_tile(int nx, int nv) {
 // nv = 1024 the separate launch is faster (23.9 vs 26.0 us per step), so the
 // tail is left to the autotuner.
 bool default_fuse(int) { return false; }
-
-hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 enum KernelKind { KK_ROW = 0, KK_DRIFT = 1, KK_FIELD = 2, KK_OTHER = 3 };
 
@@ -685,6 +597,7 @@ struct vpfp_batch_s {
   double* dens = nullptr;
   unsigned int* counters = nullptr;
   double* ering = nullptr;
+  StreamLog streams;
   std::vector<void*> allocs;
   ~vpfp_batch_s() {
     for (void* p : allocs) (void)hipFree(p);
@@ -1072,7 +985,7 @@ extern "C" {
 
 int vpfp_version(void) { return 1; }
 
-const char* vpfp_last_error(void) { return g_err.c_str(); }
+const char* vpfp_last_error(void) { return vpfp_impl::last_error(); }
 
 int vpfp_plan_create(const vpfp_grid_desc* d, vpfp_plan_t* out) {
   if (!d || !out) return fail(VPFP_EBADARG, "null argument");
@@ -1136,7 +1049,7 @@ int vpfp_plan_create(const vpfp_grid_desc* d, vpfp_plan_t* out) {
 
 int vpfp_plan_destroy(vpfp_plan_t p) {
   if (!p) return VPFP_OK;
-  (void)hipDeviceSynchronize();
+  p->streams.drain();
   delete p;
   return VPFP_OK;
 }
@@ -1159,10 +1072,10 @@ int vpfp_plan_background(vpfp_plan_t p, void* f, int sign, void* stream) {
   const size_t n = (size_t)p->nx * p->nv;
   const unsigned grid = (unsigned)((n + 255) / 256);
   if (p->prec == VPFP_F64)
-    hipLaunchKernelGGL(k_background<double>, dim3(grid), dim3(256), 0, S(stream), reinterpret_cast<double*>(f), n,
+    hipLaunchKernelGGL(k_background<double>, dim3(grid), dim3(256), 0, p->streams.use(stream), reinterpret_cast<double*>(f), n,
                        p->nv, p->bg, (double)sign);
   else
-    hipLaunchKernelGGL(k_background<float>, dim3(grid), dim3(256), 0, S(stream), reinterpret_cast<float*>(f), n,
+    hipLaunchKernelGGL(k_background<float>, dim3(grid), dim3(256), 0, p->streams.use(stream), reinterpret_cast<float*>(f), n,
                        p->nv, p->bg, (double)sign);
   VPFP_HIP(hipGetLastError());
   return VPFP_OK;
@@ -1172,24 +1085,24 @@ int vpfp_vdfdx(vpfp_plan_t p, const void* fin, void* fout, double dt, void* stre
   if (int rc = check_plan(p)) return rc;
   if (int rc = need_spectral_x(p)) return rc;
   if (!fin || !fout) return fail(VPFP_EBADARG, "null array");
-  return p->prec == VPFP_F64 ? do_vdfdx<double>(p, fin, fout, dt, nullptr, S(stream))
-                             : do_vdfdx<float>(p, fin, fout, dt, nullptr, S(stream));
+  return p->prec == VPFP_F64 ? do_vdfdx<double>(p, fin, fout, dt, nullptr, p->streams.use(stream))
+                             : do_vdfdx<float>(p, fin, fout, dt, nullptr, p->streams.use(stream));
 }
 
 int vpfp_edfdv(vpfp_plan_t p, const void* fin, void* fout, const double* e, double dt, void* stream) {
   if (int rc = check_plan(p)) return rc;
   if (int rc = need_spectral_v(p)) return rc;
   if (!fin || !fout || !e) return fail(VPFP_EBADARG, "null array");
-  return p->prec == VPFP_F64 ? do_edfdv<double>(p, fin, fout, e, dt, S(stream))
-                             : do_edfdv<float>(p, fin, fout, e, dt, S(stream));
+  return p->prec == VPFP_F64 ? do_edfdv<double>(p, fin, fout, e, dt, p->streams.use(stream))
+                             : do_edfdv<float>(p, fin, fout, e, dt, p->streams.use(stream));
 }
 
 int vpfp_compute_charges(vpfp_plan_t p, const void* f, double* n_out, void* stream) {
   if (int rc = check_plan(p)) return rc;
   if (int rc = no_delta(p, "vpfp_compute_charges")) return rc;
   if (!f || !n_out) return fail(VPFP_EBADARG, "null array");
-  return p->prec == VPFP_F64 ? do_density<double>(p, f, p->dv, n_out, S(stream))
-                             : do_density<float>(p, f, p->dv, n_out, S(stream));
+  return p->prec == VPFP_F64 ? do_density<double>(p, f, p->dv, n_out, p->streams.use(stream))
+                             : do_density<float>(p, f, p->dv, n_out, p->streams.use(stream));
 }
 
 int vpfp_solve_for_field(vpfp_plan_t p, const double* n, double* e_out, void* stream) {
@@ -1198,25 +1111,25 @@ int vpfp_solve_for_field(vpfp_plan_t p, const double* n, double* e_out, void* st
   if (p->nx < 2) return fail(VPFP_EUNSUPPORTED, "nx must be at least 2");
   const bool pow2 = p->nx_pow2 && p->nx >= 4 && p->nx <= 4096;
   auto fa = field_args(p, n, 1, 1.0, nullptr, e_out, nullptr, nullptr, pow2);
-  return field_dispatch(fa, pow2, S(stream));
+  return field_dispatch(fa, pow2, p->streams.use(stream));
 }
 
 int vpfp_field(vpfp_plan_t p, const void* f, const double* driver, double* e_out, void* stream) {
   if (int rc = check_plan(p)) return rc;
   if (!f || !e_out) return fail(VPFP_EBADARG, "null array");
-  int rc = p->prec == VPFP_F64 ? do_density<double>(p, f, p->dv, p->dens, S(stream))
-                               : do_density<float>(p, f, p->dv, p->dens, S(stream));
+  int rc = p->prec == VPFP_F64 ? do_density<double>(p, f, p->dv, p->dens, p->streams.use(stream))
+                               : do_density<float>(p, f, p->dv, p->dens, p->streams.use(stream));
   if (rc) return rc;
   const bool pow2 = p->nx_pow2 && p->nx >= 4 && p->nx <= 4096;
   auto fa = field_args(p, p->dens, 1, 1.0, driver, e_out, nullptr, nullptr, pow2);
-  return field_dispatch(fa, pow2, S(stream));
+  return field_dispatch(fa, pow2, p->streams.use(stream));
 }
 
 int vpfp_collision_diagonals(vpfp_plan_t p, const void* f, double* a, double* b, double* c, void* stream) {
   if (int rc = check_plan(p)) return rc;
   if (int rc = no_delta(p, "vpfp_collision_diagonals")) return rc;
   if (!f || !a || !b || !c) return fail(VPFP_EBADARG, "null array");
-  return p->prec == VPFP_F64 ? do_diag<double>(p, f, a, b, c, S(stream)) : do_diag<float>(p, f, a, b, c, S(stream));
+  return p->prec == VPFP_F64 ? do_diag<double>(p, f, a, b, c, p->streams.use(stream)) : do_diag<float>(p, f, a, b, c, p->streams.use(stream));
 }
 
 int vpfp_tridiag_solve(vpfp_plan_t p, const double* a, const double* b, const double* c, const void* d, void* x,
@@ -1225,8 +1138,8 @@ int vpfp_tridiag_solve(vpfp_plan_t p, const double* a, const double* b, const do
   if (int rc = no_delta(p, "vpfp_tridiag_solve")) return rc;
   if (int rc = need_spectral_v(p)) return rc;
   if (!a || !b || !c || !d || !x) return fail(VPFP_EBADARG, "null array");
-  return p->prec == VPFP_F64 ? do_tri<double>(p, a, b, c, d, x, S(stream))
-                             : do_tri<float>(p, a, b, c, d, x, S(stream));
+  return p->prec == VPFP_F64 ? do_tri<double>(p, a, b, c, d, x, p->streams.use(stream))
+                             : do_tri<float>(p, a, b, c, d, x, p->streams.use(stream));
 }
 
 int vpfp_collide(vpfp_plan_t p, const void* fin, void* fout, void* stream) {
@@ -1235,12 +1148,12 @@ int vpfp_collide(vpfp_plan_t p, const void* fin, void* fout, void* stream) {
   if (p->nu == 0.0) {
     if (fin != fout) {
       const size_t bytes = (size_t)p->nx * p->nv * (p->prec == VPFP_F64 ? 8 : 4);
-      VPFP_HIP(hipMemcpyAsync(fout, fin, bytes, hipMemcpyDeviceToDevice, S(stream)));
+      VPFP_HIP(hipMemcpyAsync(fout, fin, bytes, hipMemcpyDeviceToDevice, p->streams.use(stream)));
     }
     return VPFP_OK;
   }
   if (int rc = need_spectral_v(p)) return rc;
-  return p->prec == VPFP_F64 ? do_collide<double>(p, fin, fout, S(stream)) : do_collide<float>(p, fin, fout, S(stream));
+  return p->prec == VPFP_F64 ? do_collide<double>(p, fin, fout, p->streams.use(stream)) : do_collide<float>(p, fin, fout, p->streams.use(stream));
 }
 
 int vpfp_moments(vpfp_plan_t p, const void* f, double* moments, void* stream) {
@@ -1251,12 +1164,12 @@ int vpfp_moments(vpfp_plan_t p, const void* f, double* moments, void* stream) {
     auto a = row_args<double>(p, f, nullptr);
     a.moments = moments;
     a.flags = vp::RK_MOMENTS;
-    return row_dispatch<double>(a, S(stream), p->row_q);
+    return row_dispatch<double>(a, p->streams.use(stream), p->row_q);
   }
   auto a = row_args<float>(p, f, nullptr);
   a.moments = moments;
   a.flags = vp::RK_MOMENTS;
-  return row_dispatch<float>(a, S(stream), p->row_q);
+  return row_dispatch<float>(a, p->streams.use(stream), p->row_q);
 }
 
 int vpfp_x_modes(vpfp_plan_t p, const void* f, int n_modes, double* out, void* stream) {
@@ -1268,10 +1181,10 @@ int vpfp_x_modes(vpfp_plan_t p, const void* f, int n_modes, double* out, void* s
   const dim3 grid((total + 255) / 256);
   auto* o = reinterpret_cast<cx<double>*>(out);
   if (p->prec == VPFP_F64)
-    hipLaunchKernelGGL(k_xmodes<double>, grid, dim3(256), 0, S(stream), reinterpret_cast<const double*>(f), p->nx,
+    hipLaunchKernelGGL(k_xmodes<double>, grid, dim3(256), 0, p->streams.use(stream), reinterpret_cast<const double*>(f), p->nx,
                        p->nv, n_modes, p->tw_xd, o);
   else
-    hipLaunchKernelGGL(k_xmodes<float>, grid, dim3(256), 0, S(stream), reinterpret_cast<const float*>(f), p->nx,
+    hipLaunchKernelGGL(k_xmodes<float>, grid, dim3(256), 0, p->streams.use(stream), reinterpret_cast<const float*>(f), p->nx,
                        p->nv, n_modes, p->tw_xd, o);
   return launched();
 }
@@ -1364,10 +1277,10 @@ int vpfp_batch_background(vpfp_batch_t b, void* f, int sign, void* stream) {
   const size_t n = (size_t)b->n * b->nx * b->nv;
   const unsigned grid = (unsigned)((n + 255) / 256);
   if (b->prec == VPFP_F64)
-    hipLaunchKernelGGL(k_background<double>, dim3(grid), dim3(256), 0, S(stream), reinterpret_cast<double*>(f), n,
+    hipLaunchKernelGGL(k_background<double>, dim3(grid), dim3(256), 0, b->streams.use(stream), reinterpret_cast<double*>(f), n,
                        b->nv, b->bg, (double)sign);
   else
-    hipLaunchKernelGGL(k_background<float>, dim3(grid), dim3(256), 0, S(stream), reinterpret_cast<float*>(f), n,
+    hipLaunchKernelGGL(k_background<float>, dim3(grid), dim3(256), 0, b->streams.use(stream), reinterpret_cast<float*>(f), n,
                        b->nv, b->bg, (double)sign);
   VPFP_HIP(hipGetLastError());
   return VPFP_OK;
@@ -1375,7 +1288,7 @@ int vpfp_batch_background(vpfp_batch_t b, void* f, int sign, void* stream) {
 
 int vpfp_batch_destroy(vpfp_batch_t b) {
   if (!b) return VPFP_OK;
-  (void)hipDeviceSynchronize();
+  b->streams.drain();
   delete b;
   return VPFP_OK;
 }
@@ -1384,8 +1297,8 @@ int vpfp_batch_step_n(vpfp_batch_t b, void* f, double* e, int n_steps, double* s
   if (!b) return fail(VPFP_EBADARG, "null batch");
   if (!f || !e) return fail(VPFP_EBADARG, "null array");
   if (n_steps < 0) return fail(VPFP_EBADARG, "negative step count");
-  return b->prec == VPFP_F64 ? batch_step_n<double>(b, f, e, n_steps, series, S(stream))
-                             : batch_step_n<float>(b, f, e, n_steps, series, S(stream));
+  return b->prec == VPFP_F64 ? batch_step_n<double>(b, f, e, n_steps, series, b->streams.use(stream))
+                             : batch_step_n<float>(b, f, e, n_steps, series, b->streams.use(stream));
 }
 
 int vpfp_plan_get_tuning(vpfp_plan_t p, vpfp_tuning* t) {
@@ -1417,8 +1330,8 @@ int vpfp_plan_autotune(vpfp_plan_t p, int n_steps, void* stream, vpfp_tuning* ch
   if (n_steps < 1) return fail(VPFP_EBADARG, "n_steps must be positive");
   if (int rc = need_spectral_v(p)) return rc;
   if (int rc = need_spectral_x(p)) return rc;
-  const int rc = p->prec == VPFP_F64 ? autotune_plan<double>(p, n_steps, S(stream))
-                                     : autotune_plan<float>(p, n_steps, S(stream));
+  const int rc = p->prec == VPFP_F64 ? autotune_plan<double>(p, n_steps, p->streams.use(stream))
+                                     : autotune_plan<float>(p, n_steps, p->streams.use(stream));
   if (rc) return rc;
   return chosen ? vpfp_plan_get_tuning(p, chosen) : VPFP_OK;
 }
@@ -1449,8 +1362,8 @@ int vpfp_batch_set_tuning(vpfp_batch_t b, const vpfp_tuning* t) {
 int vpfp_batch_autotune(vpfp_batch_t b, int n_steps, void* stream, vpfp_tuning* chosen) {
   if (!b) return fail(VPFP_EBADARG, "null batch");
   if (n_steps < 1) return fail(VPFP_EBADARG, "n_steps must be positive");
-  const int rc = b->prec == VPFP_F64 ? autotune_batch<double>(b, n_steps, S(stream))
-                                     : autotune_batch<float>(b, n_steps, S(stream));
+  const int rc = b->prec == VPFP_F64 ? autotune_batch<double>(b, n_steps, b->streams.use(stream))
+                                     : autotune_batch<float>(b, n_steps, b->streams.use(stream));
   if (rc) return rc;
   return chosen ? vpfp_batch_get_tuning(b, chosen) : VPFP_OK;
 }
@@ -1484,8 +1397,8 @@ int vpfp_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, 
   if (int rc = need_spectral_v(p)) return rc;
   if (int rc = need_spectral_x(p)) return rc;
   if (p->nx > 2048) return fail(VPFP_EUNSUPPORTED, "nx too large for the fused step");
-  return p->prec == VPFP_F64 ? do_step_n<double>(p, f, e, n, driver, e_hist, moments, series, S(stream))
-                             : do_step_n<float>(p, f, e, n, driver, e_hist, moments, series, S(stream));
+  return p->prec == VPFP_F64 ? do_step_n<double>(p, f, e, n, driver, e_hist, moments, series, p->streams.use(stream))
+                             : do_step_n<float>(p, f, e, n, driver, e_hist, moments, series, p->streams.use(stream));
 }
 
 }  // extern "C"

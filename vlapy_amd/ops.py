@@ -22,14 +22,14 @@ def f_dtype(plan):
 def _check(t, shape, dtype, name):
     if not isinstance(t, torch.Tensor):
         raise TypeError("%s must be a torch tensor" % name)
-    if not t.is_cuda:
-        raise ValueError("%s must be a device tensor" % name)
     if tuple(t.shape) != tuple(shape):
         raise ValueError("%s has shape %s, expected %s" % (name, tuple(t.shape), tuple(shape)))
     if t.dtype != dtype:
         raise ValueError("%s has dtype %s, expected %s" % (name, t.dtype, dtype))
     if not t.is_contiguous():
         raise ValueError("%s must be contiguous" % name)
+    if not t.is_cuda:
+        raise ValueError("%s must be a device tensor" % name)
     return t.data_ptr()
 
 

@@ -4,9 +4,10 @@ together on one GPU.
 
 Instances are grouped by size class (nx, nv); each class is one batch of the
 C ABI (vpfp_batch_*), whose f lives in HBM as [n_inst][nx][nv].  A step of a
-class is two launches (fused kick-collide-kick rows of every instance; drift
-of every instance's columns with each instance's field solved in the same
-launch).  Classes run concurrently on separate HIP streams.
+class is three launches: the fused kick-collide-kick rows of every instance,
+the drift of every instance's columns (with per-workgroup partial densities),
+and the field solve of every instance.  Classes run concurrently on separate
+HIP streams.
 """
 
 import ctypes
@@ -95,6 +96,12 @@ class _Class:
         t = _lib.Tuning()
         _lib.check(_lib.library().vpfp_batch_autotune(self.handle, int(n_steps), ctypes.c_void_p(stream.cuda_stream),
                                                       ctypes.byref(t)))
+        return t.as_dict()
+
+    def tuning(self):
+        """Launch shapes in use (vpfp_batch_get_tuning)."""
+        t = _lib.Tuning()
+        _lib.check(_lib.library().vpfp_batch_get_tuning(self.handle, ctypes.byref(t)))
         return t.as_dict()
 
     def set_tuning(self, t):
