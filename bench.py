@@ -1,26 +1,28 @@
 #!/usr/bin/env python
-"""Benchmark of the fused VPFP time step on one MI355X.
+"""Benchmark of the VPFP time step on one MI355X.
 
-Workload (the collisional nonlinear electron plasma wave): nx=64 cells on
-[0, 2 pi/k), k=0.3; nv=1024 cells on [-6.4, 6.4]; f0 = (1 + 0.05 cos kx)
-exp(-v^2/2)/sqrt(2 pi); e0 = 0; Dougherty collisions nu=1e-3 every step;
-dt=0.05; 4000 leapfrog steps.  The GPU runs it with f stored and advected in
-FP32 and the velocity moments, Poisson solve and tridiagonal elimination in
-FP64; the plain FP64 path is timed beside it.  A step is the full reference
-timestep: kick, drift, field, kick, collision, storage moments (E history,
-n, j, T, q, v^4, v^5, f^2, f log f per x, <E^2>).
+Headline (`value`): the batched ragged parameter scan in instances/s at the
+damping-rate bar.  4096 instances, k ~ U[0.25, 0.45], amplitude
+10^U[-4, -1], nu 10^U[-5, -2] (Dougherty), nx in {32, 64, 128}, nv in
+{256, 512, 1024, 2048} drawn from default_rng(20261202); v in [-6.4, 6.4],
+f0 = (1 + amp cos kx) exp(-v^2/2)/sqrt(2 pi), e0 = 0, dt = 0.1, 2000 leapfrog
++ collision steps, the <E^2>(t) history of every instance recorded.  The
+timed configuration stores f - M in FP32 (delta-f storage) with FP64
+moments, Poisson and tridiagonal; the same scan in FP64 is timed beside it
+and is the judge: `damping_rate` gives the fraction of fitted instances
+whose damping rate agrees with the FP64 scan's within 1 %.  Plain FP32
+storage (f itself) is timed as well.
 
-    python bench.py [--steps 4000] [--warmup 200]
+Beside it: the collisional nonlinear electron plasma wave (`epw`, nx=64,
+nv=1024, Dougherty nu=1e-3, dt=0.05, 4000 steps with the storage moments) in
+FP32, FP64 and FP32 delta-f, with the row kernel's roofline; the
+high-resolution grid (`large_grid`, 1024 x 8192); the learned closure
+(`closure`).  CPU baselines: the NumPy FP64 oracle on one core and on every
+core of the host's share (one process per core), on a sample of the scan.
 
-Prints one JSON line.  `value` is phase-space cell-updates per second
-(nx * nv * steps / wall time of the timed region, inputs resident in HBM).
+    python bench.py [--scan-steps 2000]
 
-The line also carries the ragged parameter scan (`scan`): 4096 instances,
-k ~ U[0.25, 0.45], amplitude 10^U[-4, -1], nu 10^U[-5, -2] (Dougherty),
-nx in {32, 64, 128}, nv in {256, 512, 1024, 2048} drawn from
-default_rng(20261202); dt = 0.1, 2000 steps; batched FP32 advection with
-FP64 moments/tridiagonal against the same batch in FP64, instances/s and the
-per-instance damping-rate agreement of <E^2>.
+Prints one JSON line.
 """
 
 import argparse
@@ -236,91 +238,245 @@ def scan_window(p, i, e2, t):
     return 6.0, min(0.5 * t_trap, 0.5 * t_rec, t_noise, t[-1])
 
 
-def scan_run(precision, steps, delta=False):
+SCAN_PROFILE_STEPS = 100
+
+
+SCAN_TUNING = {"in": None, "out": None}  # --scan-tuning-in / --scan-tuning-out (JSON files)
+
+
+def _tuning_key(prec, delta):
+    return "%s%s" % (prec, "_delta" if delta else "")
+
+
+def scan_run(precision, steps, delta=False, graph=False, profile_steps=0, warmup=2):
+    """One timed scan of `steps` steps from the initial state (launch shapes
+    autotuned first, untimed), <E^2> recorded every step; then optionally
+    `profile_steps` more steps with per-kernel HIP events, and the same steps
+    replayed from one HIP graph of SCAN_GRAPH_STEPS steps."""
     import torch
 
     from vlapy_amd.scan import Scan, draw_parameters
 
     p = draw_parameters(SCAN_N, SCAN_SEED)
     sc = Scan(p, dt=SCAN_DT, op="dg", precision=precision, delta=delta)
-    sc.run(2, record=False)  # warmup (also first-call kernel setup)
+    sc.run(max(1, warmup), record=False)  # warmup (also first-call kernel setup)
     torch.cuda.synchronize()
     del sc
     sc = Scan(p, dt=SCAN_DT, op="dg", precision=precision, delta=delta)
-    tuning = sc.autotune(3)  # launch shapes per size class (untimed setup)
+    key = _tuning_key(precision, delta)
+    saved = None
+    if SCAN_TUNING["in"]:
+        with open(SCAN_TUNING["in"]) as fh:
+            saved = json.load(fh).get(key)
+    if saved:
+        # the launch shapes of an earlier timed run (counter runs)
+        tuning = {(c.nx, c.nv): saved["%dx%d" % (c.nx, c.nv)] for c in sc.classes}
+        sc.set_tuning(tuning)
+    else:
+        tuning = sc.autotune(3)  # launch shapes per size class (untimed setup)
+    if SCAN_TUNING["out"]:
+        old = {}
+        if os.path.exists(SCAN_TUNING["out"]):
+            with open(SCAN_TUNING["out"]) as fh:
+                old = json.load(fh)
+        old[key] = {"%dx%d" % k: v for k, v in tuning.items()}
+        with open(SCAN_TUNING["out"], "w") as fh:
+            json.dump(old, fh, indent=1)
+    n0, w0 = sc.density(), sc.energy(self_consistent=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e2 = sc.run(steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    # the same steps replayed from one HIP graph of SCAN_GRAPH_STEPS steps
-    # (from the initial state again; no <E^2> record)
-    graph_wall = None
-    if steps % SCAN_GRAPH_STEPS == 0 and not delta:
+    import numpy as np
+
+    r = {"p": p, "cells": sc.cells, "e2": e2, "wall": wall, "tuning": tuning,
+         "density_drift": np.abs(sc.density() / n0 - 1.0), "energy_drift": np.abs(sc.energy() / w0 - 1.0)}
+    if profile_steps:
+        sc.set_profiling(True)
+        sc.run(profile_steps, record=False)
+        torch.cuda.synchronize()
+        r["kernels"] = sc.kernel_times()
+        r["profile_steps"] = profile_steps
+        r["launch_bytes"] = launch_bytes(sc, tuning, 8 if precision == "fp64" else 4)
+        sc.set_profiling(False)
+    r["graph_wall"] = None
+    if graph and steps % SCAN_GRAPH_STEPS == 0:
         del sc
-        sc = Scan(p, dt=SCAN_DT, op="dg", precision=precision)
-        for c, (k, t) in zip(sc.classes, tuning.items()):
-            c.set_tuning(t)
+        sc = Scan(p, dt=SCAN_DT, op="dg", precision=precision, delta=delta)
+        sc.set_tuning(tuning)
         sc.capture(SCAN_GRAPH_STEPS)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         sc.replay(steps // SCAN_GRAPH_STEPS)
         torch.cuda.synchronize()
-        graph_wall = time.perf_counter() - t0
-    return p, sc.cells, e2, wall, tuning, graph_wall
+        r["graph_wall"] = time.perf_counter() - t0
+    return r
 
 
-def scan_cpu(p, steps_sample=10, n_sample=12):
-    """Oracle time per instance-step on a sample of instances (one thread),
-    extrapolated to the whole scan."""
+def launch_bytes(sc, tuning, s):
+    """Algorithmic bytes of one launch of each kernel of the scan's step, keyed
+    as Scan.kernel_times(): per grouped launch ("nv", nv) / ("nx", nx), or per
+    size class.  s bytes per stored value, cells of the launch's classes:
+    row kernel 3 split operators (kick, collision, kick) = 6 s cells per fused
+    launch; drift 2 s cells; field the partial densities and E,
+    8 (drift workgroups + 1) nx per instance."""
+    out = {}
+    for c in sc.classes:
+        t = tuning[(c.nx, c.nv)]
+        gx = -(-(c.nv // t["drift_tile"]) // max(1, t["drift_tiles"]))
+        cells = c.n * c.nx * c.nv
+        rk, xk = (("nv", c.nv), ("nx", c.nx)) if sc.group is not None else ((c.nx, c.nv), (c.nx, c.nv))
+        out.setdefault(rk, {}).setdefault("row", 0)
+        out.setdefault(xk, {}).setdefault("drift", 0)
+        out[xk].setdefault("field", 0)
+        out[rk]["row"] += 6 * s * cells
+        out[xk]["drift"] += 2 * s * cells
+        out[xk]["field"] += 8 * (gx + 1) * c.nx * c.n
+    return out
+
+
+def _launch_name(key):
+    return "%s=%d" % key if isinstance(key[0], str) else "nx=%d, nv=%d" % key
+
+
+def scan_roofline(run, traffic_file=None):
+    """Dominant kernel of the scan (the launch with the most event time over
+    the profiled steps): its algorithmic bytes (launch_bytes) over its
+    average launch duration, against the 8 TB/s HBM peak.  Over n profiled
+    steps a row launch is made n + 1 times carrying 3n split operators in
+    all (the first is a half kick alone, the last has no closing half kick),
+    which is what `achieved` divides."""
+    kern = run["kernels"]
+    cands = [(k, kind) for k in kern for kind in ("row", "drift", "field") if kern[k].get(kind, (0, 0))[1]]
+    key, kind = max(cands, key=lambda kk: kern[kk[0]][kk[1]][0])
+    total_ms, launches = kern[key][kind]
+    n = run["profile_steps"]
+    per_launch = run["launch_bytes"][key][kind]
+    achieved = per_launch * n / (total_ms / 1e3) / 1e9
+    traffic = None
+    if traffic_file:
+        prof = os.path.join(HERE, "profiles", traffic_file)
+        if os.path.exists(prof):
+            with open(prof) as fh:
+                t = json.load(fh)
+            if t.get("launch") == _launch_name(key):
+                traffic = t.get(kind)
+    step_ms = sum(v[0] for d in kern.values() for v in d.values()) / n
+    return {
+        "bound": "hbm",
+        "kernel": "%s (%s)" % (kind, _launch_name(key)),
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "alg_bytes_per_launch": per_launch,
+        "avg_launch_us": total_ms / launches * 1e3,
+        "launches": launches,
+        "kernel_ms_per_step": {_launch_name(k): {kk: v[0] / n for kk, v in d.items() if v[1]} for k, d in kern.items()},
+        "sum_of_kernel_ms_per_step": step_ms,
+    }
+
+
+def scan_cpu(p, steps_sample, n_sample, procs):
+    """Oracle time on a sample of the scan's instances, extrapolated by
+    cell-steps to the whole scan (run in a child process with one BLAS/OMP
+    thread per process): procs = 1 is one core; procs > 1 runs the sample
+    as one process per core over the instance list."""
     env = dict(os.environ)
-    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
         env[k] = "1"
     res = subprocess.run([sys.executable, os.path.abspath(__file__), "--scan-cpu-child",
-                          "%d,%d" % (steps_sample, n_sample)], capture_output=True, text=True, env=env, timeout=900)
+                          "%d,%d,%d" % (steps_sample, n_sample, procs)], capture_output=True, text=True, env=env,
+                         timeout=900)
     if res.returncode != 0:
         raise RuntimeError("scan cpu baseline failed: " + res.stderr[-2000:])
     return json.loads(res.stdout.strip().splitlines()[-1])
 
 
-def scan_cpu_child(arg):
+def _oracle_instance(args):
+    """Run instance i of the scan draw for `steps` steps on the oracle;
+    returns (seconds, cell-steps)."""
+    i, steps = args
     import numpy as np
 
     sys.path.insert(0, HERE)
     from oracle import vpfp_oracle as O
     from vlapy_amd.scan import draw_parameters, instance_grids
 
-    steps, n_sample = (int(a) for a in arg.split(","))
+    p = draw_parameters(SCAN_N, SCAN_SEED)
+    x, kx, okx, v, kv, dv = instance_grids(int(p["nx"][i]), p["k"][i], int(p["nv"][i]), 6.4)
+    g = O.Grid(x, kx, okx, v, kv, dv)
+    f = O.perturbed_maxwellian(x, v, p["k"][i], p["amp"][i])
+    t0 = time.perf_counter()
+    O.run_steps(g, np.zeros(x.size), f, steps, SCAN_DT, nu=p["nu"][i], fp_type="dg")
+    return time.perf_counter() - t0, steps * x.size * v.size
+
+
+def scan_cpu_child(arg):
+    import numpy as np
+
+    from vlapy_amd.scan import draw_parameters
+
+    steps, n_sample, procs = (int(a) for a in arg.split(","))
     p = draw_parameters(SCAN_N, SCAN_SEED)
     # a sample spread over the size classes: every (4096 / n_sample)-th instance
     idx = np.arange(0, SCAN_N, SCAN_N // n_sample)[:n_sample]
-    per_cell = []
-    total = 0.0
-    for i in idx:
-        x, kx, okx, v, kv, dv = instance_grids(int(p["nx"][i]), p["k"][i], int(p["nv"][i]), 6.4)
-        g = O.Grid(x, kx, okx, v, kv, dv)
-        f = O.perturbed_maxwellian(x, v, p["k"][i], p["amp"][i])
-        t0 = time.perf_counter()
-        O.run_steps(g, np.zeros(x.size), f, steps, SCAN_DT, nu=p["nu"][i], fp_type="dg")
-        el = time.perf_counter() - t0
-        total += el
-        per_cell.append(el / steps / (x.size * v.size))
-    cells_all = np.sum(p["nx"].astype(float) * p["nv"].astype(float))
-    # seconds for the whole scan at the sample's mean time per cell-step
-    est = float(np.mean(per_cell)) * cells_all * SCAN_STEPS
-    print(json.dumps({"sample_instances": int(len(idx)), "sample_steps": steps, "sample_seconds": total,
+    t0 = time.perf_counter()
+    if procs == 1:
+        res = [_oracle_instance((int(i), steps)) for i in idx]
+    else:
+        import multiprocessing as mp
+
+        with mp.get_context("fork").Pool(procs) as pool:
+            res = pool.map(_oracle_instance, [(int(i), steps) for i in idx], chunksize=1)
+    wall = time.perf_counter() - t0
+    work = float(sum(r[1] for r in res))
+    cells_all = float(np.sum(p["nx"].astype(float) * p["nv"].astype(float)))
+    # seconds for the whole scan at the sample's throughput in cell-steps/s
+    est = wall * cells_all * SCAN_STEPS / work
+    print(json.dumps({"sample_instances": int(len(idx)), "sample_steps": steps, "procs": procs,
+                      "sample_wall_s": wall, "sample_cpu_s": float(sum(r[0] for r in res)),
                       "est_scan_seconds": est}))
 
 
-def scan_section(steps, with_cpu):
+def host_cores():
+    """Cores of this host's share: the affinity mask, at most 16 (the CPU
+    share of one GPU on the benchmark host)."""
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = os.cpu_count() or 1
+    return min(aff, 16), aff
+
+
+def scan_section(timed_steps, warmup, with_cpu, judge=True, profile_steps=SCAN_PROFILE_STEPS):
+    """The headline: `timed_steps` steps of the FP32 delta-f scan from the
+    initial state (value = completed runs of SCAN_STEPS steps per second,
+    SCAN_N * timed_steps / SCAN_STEPS / wall), then the full-length runs that
+    judge it: FP32 delta-f (the same run when timed_steps == SCAN_STEPS),
+    FP64 and plain FP32, SCAN_STEPS steps each."""
     import numpy as np
 
     from vlapy_amd.diagnostics.low_level_helpers import get_standing_wave_rate_and_frequency as rate
 
-    p, cells, e2_32, wall32, tun32, gwall32 = scan_run("fp32", steps)
-    _, _, e2_64, wall64, _, _ = scan_run("fp64", steps)
-    _, _, e2_d, wall_d, _, _ = scan_run("fp32", steps, delta=True)
+    steps = SCAN_STEPS
+    head = scan_run("fp32", timed_steps, delta=True, warmup=warmup, profile_steps=max(1, profile_steps),
+                    graph=(timed_steps == SCAN_STEPS))
+    if not judge:
+        # profiling runs: the timed steps and the kernel roofline only
+        return {"headline": {"timed_steps": timed_steps, "seconds": head["wall"],
+                             "runs_per_s": SCAN_N * timed_steps / SCAN_STEPS / head["wall"],
+                             "ms_per_step": head["wall"] / timed_steps * 1e3},
+                "fp32_delta_f": {"damping_rate": None},
+                "roofline": scan_roofline(head, "traffic_scan_fp32_delta.json")}
+    rd = head if timed_steps == SCAN_STEPS else scan_run("fp32", steps, delta=True, graph=True)
+    r64 = scan_run("fp64", steps)
+    r32 = scan_run("fp32", steps, graph=True)
+    p = rd["p"]
     t = SCAN_DT * (np.arange(steps) + 1)
+    e2_64 = r64["e2"]
 
     def damping_errs(e2):
         errs = []
@@ -340,37 +496,50 @@ def scan_section(steps, with_cpu):
                 "median_rel_err": float(np.median(errs)) if errs.size else None,
                 "p95_rel_err": float(np.percentile(errs, 95)) if errs.size else None}
 
-    errs = damping_errs(e2_32)
+    def drifts(r):
+        return {"density_max": float(r["density_drift"].max()), "density_median": float(np.median(r["density_drift"])),
+                "energy_max": float(r["energy_drift"].max()), "energy_median": float(np.median(r["energy_drift"]))}
+
+    cells = rd["cells"]
     out = {
         "workload": "4096-instance ragged scan, seed 20261202, Dougherty, dt=0.1, %d steps" % steps,
-        "instances_per_s": SCAN_N / wall32,
-        "graph_instances_per_s": SCAN_N / gwall32 if gwall32 else None,
-        "cell_updates_per_s": cells * steps / wall32,
-        "seconds": wall32,
-        "fp64_instances_per_s": SCAN_N / wall64,
-        "fp64_seconds": wall64,
         "cells": cells,
+        "headline": {"timed_steps": timed_steps, "seconds": head["wall"],
+                     "runs_per_s": SCAN_N * timed_steps / SCAN_STEPS / head["wall"],
+                     "ms_per_step": head["wall"] / timed_steps * 1e3},
+        "fp32_delta_f": {"instances_per_s": SCAN_N / rd["wall"], "seconds": rd["wall"],
+                         "ms_per_step": rd["wall"] / steps * 1e3,
+                         "cell_updates_per_s": cells * steps / rd["wall"],
+                         "graph_instances_per_s": SCAN_N / rd["graph_wall"] if rd["graph_wall"] else None,
+                         "damping_rate": summary(damping_errs(rd["e2"])), "drift": drifts(rd),
+                         "tuning": {"%dx%d" % k: v for k, v in rd["tuning"].items()}},
+        "fp64": {"instances_per_s": SCAN_N / r64["wall"], "seconds": r64["wall"],
+                 "cell_updates_per_s": cells * steps / r64["wall"], "drift": drifts(r64)},
+        "fp32": {"instances_per_s": SCAN_N / r32["wall"], "seconds": r32["wall"],
+                 "graph_instances_per_s": SCAN_N / r32["graph_wall"] if r32["graph_wall"] else None,
+                 "damping_rate": summary(damping_errs(r32["e2"])), "drift": drifts(r32)},
         # algorithmic bytes of a leapfrog + collision step: 8 s per cell-update
-        "roofline": {"bound": "hbm", "achieved": 32.0 * cells * steps / wall32 / 1e9, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": 32.0 * cells * steps / wall32 / 1e9 / HBM_PEAK_GBS,
-                     "fp64_achieved": 64.0 * cells * steps / wall64 / 1e9,
-                     "fp64_frac": 64.0 * cells * steps / wall64 / 1e9 / HBM_PEAK_GBS},
-        "tuning": {"%dx%d" % k: v for k, v in tun32.items()},
-        "damping_rate": summary(errs),
-        # FP32 storage of f - M (vpfp_batch_set_delta): damping rates against FP64
-        "delta_f": {"instances_per_s": SCAN_N / wall_d, "seconds": wall_d,
-                    "damping_rate": summary(damping_errs(e2_d))},
+        "step_alg_GBps": {"fp32_delta_f": 32.0 * cells * steps / rd["wall"] / 1e9,
+                          "fp64": 64.0 * cells * steps / r64["wall"] / 1e9},
+        "roofline": scan_roofline(head, "traffic_scan_fp32_delta.json"),
     }
     if with_cpu:
-        c = scan_cpu(p)
-        est = c["est_scan_seconds"] * steps / SCAN_STEPS
+        procs, aff = host_cores()
+        one = scan_cpu(p, 25, 24, 1)
+        allc = scan_cpu(p, 30, 64, procs)
+        est1 = one["est_scan_seconds"] * steps / SCAN_STEPS
+        estn = allc["est_scan_seconds"] * steps / SCAN_STEPS
         out["cpu_baseline"] = {
-            "instances_per_s": SCAN_N / est,
-            "cores": 1,
-            "kind": "port",
-            "sample": "%d instances x %d steps of the scan on the NumPy FP64 oracle (%.1f s), extrapolated by "
-                      "cells and steps to the whole scan" % (c["sample_instances"], c["sample_steps"],
-                                                             c["sample_seconds"]),
+            "one_core": {"instances_per_s": SCAN_N / est1, "cores": 1, "kind": "port",
+                         "sample": "%d instances x %d steps of the scan on the NumPy FP64 oracle (%.1f s), "
+                                   "extrapolated by cell-steps to the whole scan" % (
+                                       one["sample_instances"], one["sample_steps"], one["sample_wall_s"])},
+            "all_cores": {"instances_per_s": SCAN_N / estn, "cores": procs, "kind": "port",
+                          "sample": "%d instances x %d steps of the scan on the NumPy FP64 oracle, one process per "
+                                    "core over the instance list (%.1f s wall, %.1f s CPU), extrapolated by "
+                                    "cell-steps to the whole scan" % (allc["sample_instances"], allc["sample_steps"],
+                                                                     allc["sample_wall_s"], allc["sample_cpu_s"])},
+            "os_cpu_count": os.cpu_count(), "affinity_cores": aff,
         }
     return out
 
@@ -500,13 +669,17 @@ def closure_section():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=NSTEPS)
-    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=SCAN_STEPS,
+                    help="timed steps of the headline scan (a full run is %d)" % SCAN_STEPS)
+    ap.add_argument("--warmup", type=int, default=2, help="untimed warmup steps of the headline scan")
+    ap.add_argument("--epw-steps", type=int, default=NSTEPS)
+    ap.add_argument("--epw-warmup", type=int, default=200)
     ap.add_argument("--profile-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--scan-steps", type=int, default=SCAN_STEPS)
-    ap.add_argument("--no-scan", action="store_true")
+    ap.add_argument("--no-judge", action="store_true",
+                    help="skip the full-length FP64 / FP32 scans that judge the damping rates (profiling runs)")
+    ap.add_argument("--no-epw", action="store_true")
     ap.add_argument("--large-steps", type=int, default=200)
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-closure", action="store_true")
@@ -514,6 +687,10 @@ def main():
                     help="keep the default launch shapes (for counter runs that should see only the timed shapes)")
     ap.add_argument("--copy-bytes", type=int, default=None,
                     help="only copy a device buffer of this many bytes 5 times (HBM counter calibration)")
+    ap.add_argument("--scan-tuning-out", default=None, help="write the scan's launch shapes to this JSON file")
+    ap.add_argument("--scan-tuning-in", default=None,
+                    help="use the scan launch shapes of this JSON file instead of autotuning (counter runs)")
+    ap.add_argument("--scan-profile-steps", type=int, default=SCAN_PROFILE_STEPS)
     ap.add_argument("--cpu-child", type=float, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--scan-cpu-child", type=str, default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -538,67 +715,85 @@ def main():
         print("bench.py measures one GPU only: the build has no multi-GPU path (the parameter scan "
               "would shard its instances over GPUs with no exchange)", file=sys.stderr)
         return 2
-
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
-    at = not args.no_autotune
-    r32 = gpu_run("fp32", args.steps, args.warmup, min(args.profile_steps, args.steps), at)
-    r64 = gpu_run("fp64", args.steps, args.warmup, min(args.profile_steps, args.steps), at)
-    r32d = gpu_run("fp32", args.steps, args.warmup, min(args.profile_steps, args.steps), at, delta=True)
+    if args.steps < 1:
+        print("--steps must be positive", file=sys.stderr)
+        return 2
 
     import numpy as np
 
-    cells = NX * NV
-    value = cells * args.steps / (r32["ms"] / 1e3)
-    e32, e64 = r32["e_hist"], r64["e_hist"]
-    err_e = float(np.abs(e32 - e64).max() / np.abs(e64).max())
-    err_f = float(np.abs(r32["f"] - r64["f"]).max() / np.abs(r64["f"]).max())
+    SCAN_TUNING["in"], SCAN_TUNING["out"] = args.scan_tuning_in, args.scan_tuning_out
+    scan = scan_section(args.steps, args.warmup, not args.no_cpu, judge=not args.no_judge,
+                        profile_steps=args.scan_profile_steps)
+    head = scan["headline"]
+    dr = scan["fp32_delta_f"]["damping_rate"]
     out = {
         "metric": METRIC,
-        "value": value,
-        "unit": "cell-updates/s",
+        "value": head["runs_per_s"],
+        "unit": "instances/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r32["ms"] / args.steps,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": WORKLOAD,
-            "precision": "f32 storage and advection FFTs; f64 moments, Poisson and tridiagonal",
-            "grid": [NX, NV],
+            "workload": "ragged parameter scan: 4096 instances, k ~ U[0.25, 0.45], amplitude 10^U[-4,-1], "
+                        "nu 10^U[-5,-2] (Dougherty), nx in {32,64,128}, nv in {256,512,1024,2048} from "
+                        "default_rng(20261202), v in [-6.4, 6.4], dt=0.1, %d steps per run, <E^2>(t) recorded; "
+                        "value = completed runs per second over the timed steps" % SCAN_STEPS,
+            "precision": "f32 storage of f - M(v) (delta-f) and f32 advection FFTs; f64 moments, Poisson and "
+                         "tridiagonal",
+            "damping_rate_vs_fp64": dr,
             "parallelism": "single GPU",
         },
-        "roofline": roofline(r32, cells, r32["ms"] / args.steps, "traffic_c2_fp32.json"),
-        "cpu_baseline": cpu,
-        "tuning": r32["tuning"],
-        "fp64": {
-            "value": cells * args.steps / (r64["ms"] / 1e3),
-            "ms_per_step": r64["ms"] / args.steps,
-            "roofline": roofline(r64, cells, r64["ms"] / args.steps, "traffic_c2_fp64.json"),
-            "tuning": r64["tuning"],
-        },
-        "f32_vs_f64_after_run": {"E_max_rel": err_e, "f_max_rel": err_f},
-        # FP32 with f - M stored (vpfp_plan_set_delta): same workload and step count
-        "fp32_delta_f": {
-            "value": cells * args.steps / (r32d["ms"] / 1e3),
-            "ms_per_step": r32d["ms"] / args.steps,
-            "vs_fp64_after_run": {
-                "E_max_rel": float(np.abs(r32d["e_hist"] - e64).max() / np.abs(e64).max()),
-                "f_max_rel": float(np.abs(r32d["f"] - r64["f"]).max() / np.abs(r64["f"]).max())},
-            "tuning": r32d["tuning"],
-        },
+        "roofline": scan["roofline"],
+        "cpu_baseline": None,
+        "scan": scan,
     }
-    if not args.no_scan:
-        out["scan"] = scan_section(args.scan_steps, not args.no_cpu)
+    if "cpu_baseline" in scan:
+        a = scan["cpu_baseline"]["all_cores"]
+        out["cpu_baseline"] = {"value": a["instances_per_s"], "unit": "instances/s", "cores": a["cores"],
+                               "kind": a["kind"], "sample": a["sample"],
+                               "one_core_value": scan["cpu_baseline"]["one_core"]["instances_per_s"]}
+    if not args.no_epw:
+        out["epw"] = epw_section(args.epw_steps, args.epw_warmup, min(args.profile_steps, args.epw_steps),
+                                 not args.no_autotune, None if args.no_cpu else args.cpu_seconds)
     if not args.no_large:
-        out["large_grid"] = large_section(args.large_steps, at)
+        out["large_grid"] = large_section(args.large_steps, not args.no_autotune)
     if not args.no_closure:
         out["closure"] = closure_section()
     print(json.dumps(out))
     return 0
+
+
+def epw_section(steps, warmup, profile_steps, autotune, cpu_seconds):
+    """The collisional EPW (nx=64, nv=1024) in FP32, FP64 and FP32 delta-f,
+    each `steps` fused steps with the storage moments."""
+    import numpy as np
+
+    cpu = cpu_baseline(cpu_seconds) if cpu_seconds else None
+    r32 = gpu_run("fp32", steps, warmup, profile_steps, autotune)
+    r64 = gpu_run("fp64", steps, warmup, profile_steps, autotune)
+    r32d = gpu_run("fp32", steps, warmup, profile_steps, autotune, delta=True)
+    cells = NX * NV
+    e64 = r64["e_hist"]
+
+    def err(r):
+        return {"E_max_rel": float(np.abs(r["e_hist"] - e64).max() / np.abs(e64).max()),
+                "f_max_rel": float(np.abs(r["f"] - r64["f"]).max() / np.abs(r64["f"]).max())}
+
+    def line(r, traffic):
+        return {"cell_updates_per_s": cells * steps / (r["ms"] / 1e3), "ms_per_step": r["ms"] / steps,
+                "roofline": roofline(r, cells, r["ms"] / steps, traffic), "tuning": r["tuning"]}
+
+    return {"workload": WORKLOAD, "steps": steps,
+            "fp32": dict(line(r32, "traffic_c2_fp32.json"), vs_fp64_after_run=err(r32)),
+            "fp64": line(r64, "traffic_c2_fp64.json"),
+            "fp32_delta_f": dict(line(r32d, None), vs_fp64_after_run=err(r32d)),
+            "cpu_baseline": cpu}
 
 
 if __name__ == "__main__":

@@ -126,6 +126,19 @@ int vpfp_collide(vpfp_plan_t plan, const void* f_in, void* f_out, void* stream);
 int vpfp_step_n(vpfp_plan_t plan, void* f, double* e, int n_steps, const double* driver, double* e_hist,
                 double* moments, double* series, void* stream);
 
+/* vpfp_step_n with the reference's per-step f store (get_f_update,
+ * step.py:116-135, as the inner loop keeps it in temp_storage["stored_f"],
+ * outer_loop.py:149-180): after the collision of step i, f_store[i] receives
+ *   n_modes == 0: a copy of f, f_store [n_steps][nx][nv] in f's storage type
+ *                 (written by the row kernel between the collision and the
+ *                 next half kick, no extra launch);
+ *   n_modes > 0:  the first n_modes Fourier modes along x of f, f_store
+ *                 [n_steps][n_modes][nv] complex128 as vpfp_x_modes (one
+ *                 extra launch per step on a copy the row kernel writes).
+ * Under delta-f storage the store holds f itself. */
+int vpfp_step_n_store(vpfp_plan_t plan, void* f, double* e, int n_steps, const double* driver, double* e_hist,
+                      double* moments, double* series, void* f_store, int n_modes, void* stream);
+
 /* Trapz moments of f per x after the step: moments [VPFP_NMOMENTS][nx] =
  * n, j, T, q, fv4, vN (integrals of f v^p, p = 0..5) then f^2 and f log f.
  * Replaces update_fields / the f-integrals of update_series, step.py:153-224. */
@@ -162,6 +175,39 @@ typedef struct {
 int vpfp_batch_create(const vpfp_batch_desc* desc, vpfp_batch_t* out);
 int vpfp_batch_step_n(vpfp_batch_t batch, void* f, double* e, int n_steps, double* series, void* stream);
 int vpfp_batch_destroy(vpfp_batch_t batch);
+
+/* Several batches stepped together (no counterpart in the reference): the
+ * size classes of one ragged scan.  A step is one row launch per distinct
+ * nv (kick, collision, kick of every row of every batch with that nv), one
+ * drift and one field launch per distinct nx, each launch taking a table of
+ * the batches concerned (at most 8 batches per nv and per nx).  The batches
+ * must share precision, collision operator, dt and delta-f storage, batches
+ * of one nv the v grid, and none may have a closure.  f, e and series are
+ * host arrays of device pointers, one per batch in the order given at
+ * creation (series entries may be NULL, or series itself); each batch's
+ * arrays are as in vpfp_batch_step_n, and the results equal those of
+ * vpfp_batch_step_n per batch with the same launch shapes.  The group keeps
+ * references to the batches: destroy it before them.
+ * vpfp_group_autotune times the row shape of each nv and the drift tile of
+ * each nx on scratch state it allocates and keeps the fastest;
+ * vpfp_group_get_shapes reports them per batch (row values per thread, drift
+ * columns per tile, tiles per drift workgroup). */
+typedef struct vpfp_group_s* vpfp_group_t;
+int vpfp_group_create(vpfp_batch_t const* batches, int n_batches, vpfp_group_t* out);
+int vpfp_group_step_n(vpfp_group_t group, void* const* f, double* const* e, int n_steps, double* const* series,
+                      void* stream);
+int vpfp_group_autotune(vpfp_group_t group, int n_steps, void* stream, double* us_per_step /* or NULL */);
+int vpfp_group_get_shapes(vpfp_group_t group, int* row_q, int* drift_tile, int* drift_tiles);
+/* set the shapes: each nv's row shape and each nx's drift tile are taken
+ * from the first batch (in creation order) of that nv / nx */
+int vpfp_group_set_shapes(vpfp_group_t group, const int* row_q, const int* drift_tile, const int* drift_tiles);
+/* per-kernel timing as vpfp_plan_set_profiling; vpfp_group_launch_times
+ * gives one launch of a step: kind 0 row (index = the nv's order of first
+ * appearance), 1 drift, 2 field (index = the nx's order) */
+int vpfp_group_set_profiling(vpfp_group_t group, int on);
+int vpfp_group_kernel_times(vpfp_group_t group, double* ms /*[4]*/, long long* counts /*[4]*/);
+int vpfp_group_launch_times(vpfp_group_t group, int kind, int index, double* ms, long long* count);
+int vpfp_group_destroy(vpfp_group_t group);
 
 /* Delta-f storage (no counterpart in the reference, which stores f itself).
  * With on = 1, the f arrays of vpfp_batch_step_n hold f - M(v), M the unit
@@ -222,6 +268,9 @@ int vpfp_batch_autotune(vpfp_batch_t batch, int n_steps, void* stream, vpfp_tuni
  * partial density), [2] field, [3] other. */
 int vpfp_plan_set_profiling(vpfp_plan_t plan, int on);
 int vpfp_plan_kernel_times(vpfp_plan_t plan, double* ms /*[4]*/, long long* counts /*[4]*/);
+/* The same for a batch (vpfp_batch_step_n); [3] is the attached closure. */
+int vpfp_batch_set_profiling(vpfp_batch_t batch, int on);
+int vpfp_batch_kernel_times(vpfp_batch_t batch, double* ms /*[4]*/, long long* counts /*[4]*/);
 
 const char* vpfp_last_error(void);
 int vpfp_version(void);

@@ -141,7 +141,7 @@ def test_closure_hook_refuses_unsupported_batches():
          "nv": np.array([256])}
     for precision, nv, ok in (("fp64", 256, False), ("fp32", 512, False), ("fp32", 256, True)):
         q = dict(p, nv=np.array([nv]))
-        sc = Scan(q, dt=0.1, op="dg", precision=precision)
+        sc = Scan(q, dt=0.1, op="dg", precision=precision, grouped=False)
         h = sc.classes[0].handle
         if ok:
             _lib.check(lib.vpfp_batch_set_closure(h, net.handle))
@@ -149,3 +149,7 @@ def test_closure_hook_refuses_unsupported_batches():
         else:
             with pytest.raises(NotImplementedError):
                 _lib.check(lib.vpfp_batch_set_closure(h, net.handle))
+    # a batch that belongs to a group (grouped steps run no closure)
+    sc = Scan(p, dt=0.1, op="dg", precision="fp32")
+    with pytest.raises(NotImplementedError):
+        _lib.check(lib.vpfp_batch_set_closure(sc.classes[0].handle, net.handle))

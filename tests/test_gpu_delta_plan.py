@@ -59,8 +59,11 @@ def test_plan_delta_f_fp64_reproduces_plain_plan(row_q):
     print("fp64 delta-f vs plain: E %.2e moments %.2e f %.2e" % (_rel(ehd, eh), _rel(mod[:, :6], mo[:, :6]),
                                                                 _rel(fd, f)))
     assert _rel(ehd, eh) < 1e-11
+    # the v^4 and v^5 moments weight the velocity tails, where an absolute
+    # rounding difference of ~1e-17 in f is ~1e-8 of f itself, by up to
+    # 6.4^5 ~ 1e4: 5e-11 for them (measured 1.3e-11 at Q = 8), 1e-11 otherwise
     for q in range(8):
-        assert _rel(mod[:, q], mo[:, q]) < 1e-11, q
+        assert _rel(mod[:, q], mo[:, q]) < (5e-11 if q in (4, 5) else 1e-11), q
     assert _rel(fd, f) < 1e-12
 
 

@@ -198,6 +198,47 @@ def test_step_n_matches_oracle_loop(dev, nx, nv, nu, op, n):
     assert rel(e.cpu().numpy(), ef) < 1e-11
 
 
+@pytest.mark.parametrize("nu,delta,n_modes", [(1e-2, False, 0), (1e-2, False, 3), (0.0, False, 0), (1e-2, True, 0),
+                                               (1e-2, True, 2)])
+def test_step_n_store_matches_stepwise_copies(dev, nu, delta, n_modes):
+    """vpfp_step_n_store over n steps equals n single steps of vpfp_step_n
+    with f (or its first x-modes) taken after each: the copy the row kernel
+    writes between the collision and the next half kick is f after the step
+    (f itself under delta-f storage)."""
+    import torch
+    from vlapy_amd import ops
+
+    nx, nv, k, dt, n = 32, 256, 0.35, 0.1, 6
+    print("step_n_store nu", nu, "delta", delta, "modes", n_modes, flush=True)
+    g = O.Grid.make(nx, nv, 2 * np.pi / k, 6.0)
+    f0 = O.perturbed_maxwellian(g.x, g.v, k, 0.2)
+    m = np.exp(-g.v ** 2 / 2.0) / np.sqrt(2 * np.pi)
+    ref = make_plan(g, nx, nv, nu=nu, dt=dt, op="dg")
+    f = T(f0, dev)
+    e = torch.zeros(nx, dtype=torch.float64, device=dev)
+    want = []
+    for i in range(n):
+        ops.step_n(ref, f, e, 1)
+        if n_modes:
+            out = torch.zeros(n_modes, nv, dtype=torch.complex128, device=dev)
+            want.append(ops.x_modes(ref, f, n_modes, out).cpu().numpy())
+        else:
+            want.append(f.cpu().numpy().copy())
+    p = make_plan(g, nx, nv, nu=nu, dt=dt, op="dg")
+    if delta:
+        p.set_delta(True)
+    f2 = T(f0 - m[None, :] if delta else f0, dev)
+    e2 = torch.zeros(nx, dtype=torch.float64, device=dev)
+    if n_modes:
+        st = torch.zeros(n, n_modes, nv, dtype=torch.complex128, device=dev)
+    else:
+        st = torch.zeros(n, nx, nv, dtype=torch.float64, device=dev)
+    ops.step_n_store(p, f2, e2, n, st, n_modes)
+    got = st.cpu().numpy()
+    for i in range(n):
+        assert rel(got[i], want[i]) < (1e-12 if delta else 1e-15), i
+
+
 def test_unsupported_sizes_raise(dev):
     import torch
     from vlapy_amd import ops

@@ -120,12 +120,10 @@ def test_scan_delta_f_matches_oracle(precision, tol_e2, tol_f, row_q):
     p = _params()
     n, dt = 20, 0.1
     sc = Scan(p, dt=dt, op="dg", precision=precision, delta=True)
-    for c in sc.classes:
-        t = c.tuning()
+    tun = sc.shapes()
+    for t in tun.values():
         t["row_q"] = row_q
-        if row_q == 4 and c.nv > 4096:
-            continue
-        c.set_tuning(t)
+    sc.set_tuning(tun)
     e2 = sc.run(n)
     for i in range(p["k"].size):
         f_o, e2_o = _oracle(p, i, n, dt)
@@ -167,3 +165,49 @@ def test_scan_delta_f_refuses_closure_and_round_trips_background():
     _lib.check(lib.vpfp_batch_background(c64.handle, ctypes.c_void_p(f.data_ptr()), 1, st))
     torch.cuda.synchronize()
     assert float((f - c64.f).abs().max()) < 1e-15
+
+
+@pytest.mark.parametrize("precision,delta", [("fp64", False), ("fp32", False), ("fp32", True)])
+def test_grouped_step_equals_per_class_batches(precision, delta):
+    """The grouped step (one row launch per nv, one drift and one field launch
+    per nx, vpfp_group_step_n) equals the per-class batches (vpfp_batch_step_n)
+    bit for bit when both use the same launch shapes: the kernels run the
+    same arithmetic per row and per column tile."""
+    import torch
+
+    from vlapy_amd.scan import Scan
+
+    p = _params()
+    shape = {"row_q": 8, "drift_tile": 16, "fuse_field": 0, "drift_order": 0, "drift_tiles": 2, "us_per_step": 0.0}
+    runs = []
+    for grouped in (True, False):
+        sc = Scan(p, dt=0.1, op="dg", precision=precision, delta=delta, grouped=grouped)
+        sc.set_tuning({(c.nx, c.nv): dict(shape) for c in sc.classes})
+        e2 = sc.run(12)
+        torch.cuda.synchronize()
+        runs.append((e2, [c.f.clone() for c in sc.classes], [c.e.clone() for c in sc.classes]))
+    (e2g, fg, eg), (e2c, fc, ec) = runs
+    assert np.array_equal(e2g, e2c)
+    for a, b in zip(fg, fc):
+        assert torch.equal(a, b)
+    for a, b in zip(eg, ec):
+        assert torch.equal(a, b)
+
+
+def test_grouped_scan_launches_per_step():
+    """A grouped step makes one row launch per distinct nv and one drift and
+    one field launch per distinct nx: 10 launches for the 4 x 3 size classes
+    of the scan configuration (kernel_times counts them)."""
+    from vlapy_amd.scan import Scan
+
+    p = {"k": np.full(12, 0.3), "amp": np.full(12, 1e-3), "nu": np.full(12, 1e-4),
+         "nx": np.repeat([32, 64, 128], 4), "nv": np.tile([256, 512, 1024, 2048], 3)}
+    sc = Scan(p, dt=0.1, precision="fp32", delta=True)
+    assert len(sc.classes) == 12
+    sc.run(1, record=False)
+    sc.set_profiling(True)
+    sc.run(5, record=False)
+    kt = sc.kernel_times()
+    launches = sum(v[1] for d in kt.values() for v in d.values())
+    # 5 steps: one opening half-kick row launch per nv, then 10 per step
+    assert launches == 4 + 5 * 10, kt

@@ -96,12 +96,13 @@ def test_autotune_picks_a_valid_shape_and_leaves_state_alone():
     assert rel(f.double().cpu().numpy(), f_ref) < 2e-6
 
 
-def test_batch_autotune_keeps_parity():
+@pytest.mark.parametrize("grouped", [True, False])
+def test_batch_autotune_keeps_parity(grouped):
     from vlapy_amd.scan import Scan, instance_grids
 
     p = {"k": np.array([0.3, 0.4, 0.35]), "amp": np.array([1e-2, 1e-3, 5e-2]),
          "nu": np.array([1e-3, 1e-4, 1e-2]), "nx": np.array([32, 32, 64]), "nv": np.array([256, 256, 512])}
-    sc = Scan(p, dt=0.1, op="dg", precision="fp64")
+    sc = Scan(p, dt=0.1, op="dg", precision="fp64", grouped=grouped)
     tun = sc.autotune(n_steps=3)
     print("batch tuning", tun, flush=True)
     assert set(tun) == {(32, 256), (64, 512)}

@@ -391,3 +391,16 @@ def test_oracle_driven_landau_meets_reference_integrated_test():
                       stuff["driver_function"])
     rate = llh.get_damping_rate(np.asarray(ts["fields"]["e"]), t=stuff["t"])
     np.testing.assert_almost_equal(rate, ap["nu_ld"], decimal=4)
+
+
+def test_c2_fixture_is_the_oracle_run():
+    """The stored 4000-step EPW run (tests/golden/make_c2_oracle.py) starts
+    as a fresh oracle run does: first 60 steps of <E^2> to 1e-14, initial
+    density to rounding."""
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_oracle_4000.npz"))
+    k, nx, nv, dt = 0.3, 64, 1024, 0.05
+    g = O.Grid.make(nx, nv, 2 * np.pi / k, 6.4)
+    f0 = O.perturbed_maxwellian(g.x, g.v, k, 0.05)
+    _, _, eh = O.run_steps(g, np.zeros(nx), f0, 60, dt, nu=1e-3, fp_type="dg")
+    np.testing.assert_allclose(np.mean(eh ** 2, axis=1), gold["mean_e2"][:60], rtol=1e-14, atol=0)
+    np.testing.assert_allclose(O.compute_charges(f0, g.dv), gold["n_initial"], rtol=1e-15)

@@ -85,6 +85,7 @@ SIGNATURES = {
     "vpfp_tridiag_solve": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
     "vpfp_collide": [_c_p, _c_p, _c_p, _c_p],
     "vpfp_step_n": [_c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p],
+    "vpfp_step_n_store": [_c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i, _c_p],
     "vpfp_moments": [_c_p, _c_p, _c_p, _c_p],
     "vpfp_x_modes": [_c_p, _c_p, _c_i, _c_p, _c_p],
     "vpfp_batch_create": [ctypes.POINTER(BatchDesc), ctypes.POINTER(_c_p)],
@@ -96,6 +97,17 @@ SIGNATURES = {
     "vpfp_batch_background": [_c_p, _c_p, _c_i, _c_p],
     "vpfp_plan_set_profiling": [_c_p, _c_i],
     "vpfp_plan_kernel_times": [_c_p, ctypes.POINTER(_c_d), ctypes.POINTER(ctypes.c_longlong)],
+    "vpfp_batch_set_profiling": [_c_p, _c_i],
+    "vpfp_group_create": [ctypes.POINTER(_c_p), _c_i, ctypes.POINTER(_c_p)],
+    "vpfp_group_step_n": [_c_p, ctypes.POINTER(_c_p), ctypes.POINTER(_c_p), _c_i, ctypes.POINTER(_c_p), _c_p],
+    "vpfp_group_autotune": [_c_p, _c_i, _c_p, ctypes.POINTER(_c_d)],
+    "vpfp_group_get_shapes": [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)],
+    "vpfp_group_set_shapes": [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)],
+    "vpfp_group_launch_times": [_c_p, _c_i, _c_i, ctypes.POINTER(_c_d), ctypes.POINTER(ctypes.c_longlong)],
+    "vpfp_group_set_profiling": [_c_p, _c_i],
+    "vpfp_group_kernel_times": [_c_p, ctypes.POINTER(_c_d), ctypes.POINTER(ctypes.c_longlong)],
+    "vpfp_group_destroy": [_c_p],
+    "vpfp_batch_kernel_times": [_c_p, ctypes.POINTER(_c_d), ctypes.POINTER(ctypes.c_longlong)],
     "vpfp_plan_get_tuning": [_c_p, ctypes.POINTER(Tuning)],
     "vpfp_plan_set_tuning": [_c_p, ctypes.POINTER(Tuning)],
     "vpfp_plan_autotune": [_c_p, _c_i, _c_p, ctypes.POINTER(Tuning)],

@@ -198,7 +198,8 @@ def _scan(inst, nv, precision, delta=False):
 
     n = inst["k"].size
     p = dict(inst, nx=np.full(n, NX), nv=np.full(n, nv))
-    return Scan(p, vmax=VMAX, dt=DT, op="dg", precision=precision, delta=delta)
+    # one size class: per-class batches (the closure hook attaches to a batch)
+    return Scan(p, vmax=VMAX, dt=DT, op="dg", precision=precision, delta=delta, grouped=False)
 
 
 def background(dtype=np.float32):

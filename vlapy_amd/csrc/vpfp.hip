@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -66,6 +67,24 @@ __global__ __launch_bounds__(kFT) void k_field(vp::FieldArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   VPFP_CTX;
   vp::field_body<M, Q, kFT>(c, lds, vp::field_instance(a, c.bidy));
+}
+
+// several batches of one nx in one launch (vpfp_group_step_n): one
+// workgroup per instance, the segment giving the batch's arrays
+template <int M, int Q>
+__global__ __launch_bounds__(kFT) void k_field_multi(vp::FieldMulti m) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int s = vp::seg_of(m.seg, m.nseg, (int)blockIdx.x);
+  const int inst = (int)blockIdx.x - m.seg[s].blk0;
+  vp::FieldArgs a = m.base;
+  a.part = m.seg[s].part;
+  a.scale = m.seg[s].scale;
+  a.ntiles = m.seg[s].ntiles;
+  a.okx = m.seg[s].okx;
+  a.e_out = m.seg[s].e_out;
+  a.series = m.seg[s].series;
+  GpuCtx c{(int)threadIdx.x, (int)blockDim.x, 0, 1, inst};
+  vp::field_body<M, Q, kFT>(c, lds, vp::field_instance(a, inst));
 }
 
 __global__ __launch_bounds__(kFT) void k_field_dft(vp::FieldArgs a) {
@@ -164,6 +183,34 @@ int drift_dispatch(const vp::DriftArgs<T>& a, int C, int gx, hipStream_t s, int 
   return vpfp_impl::drift_dispatch(a, C, gx, s, gy);
 }
 
+template <int M>
+int launch_field_multi(vp::FieldMulti m, hipStream_t s) {
+  constexpr int Q = (M / kFT) > 8 ? (M / kFT) : (M < 8 ? M : 8);
+  using Cfg = vp::FieldCfg<M, Q, kFT>;
+  auto kern = k_field_multi<M, Q>;
+  if (int rc = set_lds(kern, Cfg::lds)) return rc;
+  long long blocks = 0;
+  for (int i = 0; i < m.nseg; ++i) {
+    m.seg[i].blk0 = (int)blocks;
+    blocks += m.seg[i].n_inst;
+  }
+  hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(kFT), Cfg::lds, s, m);
+  return launched();
+}
+
+int field_multi_dispatch(const vp::FieldMulti& m, hipStream_t s) {
+  const int M = m.base.nx / 2;
+#define CALL(mm) launch_field_multi<mm>(m, s)
+  VPFP_SWITCH_M(M, CALL, 2048)
+#undef CALL
+  return fail(VPFP_EUNSUPPORTED, "grouped field solve needs nx a power of two in [4, 4096]");
+}
+
+template <class T>
+int row_multi_dispatch(const vp::RowMulti<T>& m, hipStream_t s, int q) {
+  return m.base.bg_hat ? row_multi_dispatch_bg(m, s, q) : row_multi_dispatch_plain(m, s, q);
+}
+
 // n_inst > 0: a batch of n_inst instance-major field solves (pow2 only)
 int field_dispatch(const vp::FieldArgs& a, bool pow2, hipStream_t s, int n_inst = 0) {
   if (pow2) {
@@ -210,6 +257,37 @@ bool uniform_grid(const double* v, int n, double* v0, double* dvv) {
 }
 
 }  // namespace
+
+// Per-kernel timing (vpfp_plan_set_profiling, vpfp_batch_set_profiling):
+// with `on`, every launch of a step is bracketed by two HIP events on its
+// stream; collect() waits for the last one and adds the elapsed times per
+// kernel class: [0] row, [1] drift, [2] field, [3] other.
+// A launch is marked with an id = kind + 4 * index (index: which launch of
+// that kind in a grouped step); ms / n sum over ids by kind, ms_id / n_id
+// keep each id.
+struct Prof {
+  bool on = false;
+  double ms[4] = {0, 0, 0, 0};
+  long long n[4] = {0, 0, 0, 0};
+  std::vector<double> ms_id;
+  std::vector<long long> n_id;
+  std::vector<hipEvent_t> ev;  // pool, 2 per timed launch
+  std::vector<int> kind;
+  size_t used = 0;
+  void reset(bool enable) {
+    on = enable;
+    for (int k = 0; k < 4; ++k) {
+      ms[k] = 0.0;
+      n[k] = 0;
+    }
+    ms_id.clear();
+    n_id.clear();
+    used = 0;
+  }
+  ~Prof() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+};
 
 // Streams an object enqueued work on.  Destroying the object waits for
 // them (its buffers must outlive that work); after more than kMax distinct
@@ -263,17 +341,11 @@ struct vpfp_plan_s {
   double* rho_g = nullptr;       // [nx] field reduction scratch
   unsigned int* counter = nullptr;
   double* series = nullptr;      // [2]
-  // per-kernel event timing (vpfp_plan_set_profiling)
-  bool profiling = false;
-  double kernel_ms[4] = {0, 0, 0, 0};  // row, drift, field, other
-  long long kernel_n[4] = {0, 0, 0, 0};
-  std::vector<hipEvent_t> ev;          // pool, 2 per timed launch
-  std::vector<int> ev_kind;
-  size_t ev_used = 0;
+  void* fcopy = nullptr;         // [nx][nv] storage type: f after a step (vpfp_step_n_store's x-modes)
+  Prof prof;  // per-kernel event timing (vpfp_plan_set_profiling)
   StreamLog streams;
   std::vector<void*> allocs;
   ~vpfp_plan_s() {
-    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
   }
 };
@@ -361,42 +433,48 @@ enum KernelKind { KK_ROW = 0, KK_DRIFT = 1, KK_FIELD = 2, KK_OTHER = 3 };
 
 // In profiling mode every launch of the fused step is bracketed by two
 // events; vpfp_step_n reads them back once its stream has drained.
-int prof_mark(vpfp_plan_t p, int kind, hipStream_t s) {
-  if (!p->profiling) return VPFP_OK;
-  if (p->ev_used + 1 >= p->ev.size()) {
-    const size_t grow = p->ev.size() + 256;
-    while (p->ev.size() < grow) {
+int prof_mark(Prof& pr, int kind, hipStream_t s) {
+  if (!pr.on) return VPFP_OK;
+  if (pr.used + 1 >= pr.ev.size()) {
+    const size_t grow = pr.ev.size() + 256;
+    while (pr.ev.size() < grow) {
       hipEvent_t e;
       VPFP_HIP(hipEventCreate(&e));
-      p->ev.push_back(e);
+      pr.ev.push_back(e);
     }
-    p->ev_kind.resize(p->ev.size() / 2 + 1);
+    pr.kind.resize(pr.ev.size() / 2 + 1);
   }
-  if ((p->ev_used & 1) == 0) p->ev_kind[p->ev_used / 2] = kind;
-  VPFP_HIP(hipEventRecord(p->ev[p->ev_used], s));
-  ++p->ev_used;
+  if ((pr.used & 1) == 0) pr.kind[pr.used / 2] = kind;
+  VPFP_HIP(hipEventRecord(pr.ev[pr.used], s));
+  ++pr.used;
   return VPFP_OK;
 }
 
-int prof_collect(vpfp_plan_t p) {
-  if (!p->profiling || p->ev_used == 0) return VPFP_OK;
-  VPFP_HIP(hipEventSynchronize(p->ev[p->ev_used - 1]));
-  for (size_t i = 0; i + 1 < p->ev_used; i += 2) {
+int prof_collect(Prof& pr) {
+  if (!pr.on || pr.used == 0) return VPFP_OK;
+  VPFP_HIP(hipEventSynchronize(pr.ev[pr.used - 1]));
+  for (size_t i = 0; i + 1 < pr.used; i += 2) {
     float ms = 0.f;
-    VPFP_HIP(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
-    const int k = p->ev_kind[i / 2];
-    p->kernel_ms[k] += ms;
-    p->kernel_n[k] += 1;
+    VPFP_HIP(hipEventElapsedTime(&ms, pr.ev[i], pr.ev[i + 1]));
+    const int id = pr.kind[i / 2];
+    pr.ms[id % 4] += ms;
+    pr.n[id % 4] += 1;
+    if ((size_t)id >= pr.ms_id.size()) {
+      pr.ms_id.resize((size_t)id + 1, 0.0);
+      pr.n_id.resize((size_t)id + 1, 0);
+    }
+    pr.ms_id[id] += ms;
+    pr.n_id[id] += 1;
   }
-  p->ev_used = 0;
+  pr.used = 0;
   return VPFP_OK;
 }
 
 #define PROF(kind, call)                                  \
   do {                                                    \
-    if (int rc_ = prof_mark(p, kind, s)) return rc_;      \
-    if (int rc_ = (call)) return rc_;                     \
-    if (int rc_ = prof_mark(p, kind, s)) return rc_;      \
+    if (int rc_ = prof_mark(p->prof, kind, s)) return rc_; \
+    if (int rc_ = (call)) return rc_;                      \
+    if (int rc_ = prof_mark(p->prof, kind, s)) return rc_; \
   } while (0)
 
 template <class T>
@@ -530,9 +608,12 @@ int do_tri(vpfp_plan_t p, const double* A, const double* B, const double* Cc, co
   return tri_dispatch<T>(a, s);
 }
 
+// f_store (optional): per-step f store; n_modes == 0: [n][nx][nv] copies of
+// f (type T), n_modes > 0: [n][n_modes][nv] complex128 x-modes of f (from
+// the row kernel's copy in p->fcopy)
 template <class T>
 int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, double* e_hist, double* moments,
-              double* series, hipStream_t s) {
+              double* series, hipStream_t s, void* f_store = nullptr, int n_modes = 0) {
   const bool coll = p->nu > 0.0;
   const double half = 0.5 * p->dt;
   auto ra = row_args<T>(p, f, f);
@@ -571,10 +652,22 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
     rb.moments = moments ? moments + (size_t)i * vp::kMoments * p->nx : nullptr;
     rb.flags = vp::RK_KICK_A | (coll ? vp::RK_COLLIDE : 0) | (moments ? vp::RK_MOMENTS : 0) |
                (i + 1 < n ? vp::RK_KICK_B : 0);
+    if (f_store)
+      rb.f_copy = n_modes > 0 ? reinterpret_cast<T*>(p->fcopy)
+                              : reinterpret_cast<T*>(f_store) + (size_t)i * p->nx * p->nv;
     PROF(KK_ROW, row_dispatch<T>(rb, s, p->row_q));
+    if (f_store && n_modes > 0) {
+      auto* o = reinterpret_cast<cx<double>*>(f_store) + (size_t)i * n_modes * p->nv;
+      const int total = n_modes * p->nv;
+      if (int rc = prof_mark(p->prof, KK_OTHER, s)) return rc;
+      hipLaunchKernelGGL(k_xmodes<T>, dim3((total + 255) / 256), dim3(256), 0, s,
+                         reinterpret_cast<const T*>(p->fcopy), p->nx, p->nv, n_modes, p->tw_xd, o);
+      if (int rc = launched()) return rc;
+      if (int rc = prof_mark(p->prof, KK_OTHER, s)) return rc;
+    }
   }
   if (n > 0) VPFP_HIP(hipMemcpyAsync(e, e_cur, sizeof(double) * p->nx, hipMemcpyDeviceToDevice, s));
-  return prof_collect(p);
+  return prof_collect(p->prof);
 }
 
 }  // namespace
@@ -587,6 +680,7 @@ struct vpfp_batch_s {
   int dtiles = 1;
   double tuned_us = 0.0;
   vpfc_net_t closure = nullptr;  // learned closure after every step (FP32, nv 256)
+  int groups = 0;                // groups holding this batch (vpfp_group_create)
   int delta = 0;                 // f stored as f - M(v)
   double* bg = nullptr;          // M(v_m) [nv] and its unnormalised DFT [nv]
   void* bg_hat = nullptr;        // (cx<double> or cx<float>: the storage precision)
@@ -597,6 +691,7 @@ struct vpfp_batch_s {
   double* dens = nullptr;
   unsigned int* counters = nullptr;
   double* ering = nullptr;
+  Prof prof;  // per-kernel event timing (vpfp_batch_set_profiling)
   StreamLog streams;
   std::vector<void*> allocs;
   ~vpfp_batch_s() {
@@ -657,8 +752,14 @@ int batch_step_n(vpfp_batch_s* b, void* f, double* e, int n, double* series, hip
   ra.dt_a = ra.dt_b = 0.5 * b->dt;
   ra.e = e;
   ra.flags = vp::RK_KICK_A;
-  if (n > 0)
-    if (int rc = row_dispatch<T>(ra, s, b->row_q)) return rc;
+  Prof& pr = b->prof;
+#define BPROF(kind, call)                                 \
+  do {                                                    \
+    if (int rc_ = prof_mark(pr, kind, s)) return rc_;     \
+    if (int rc_ = (call)) return rc_;                     \
+    if (int rc_ = prof_mark(pr, kind, s)) return rc_;     \
+  } while (0)
+  if (n > 0) BPROF(KK_ROW, row_dispatch<T>(ra, s, b->row_q));
   const double* e_cur = e;
   const bool closure = b->closure != nullptr;
   for (int i = 0; i < n; ++i) {
@@ -666,7 +767,7 @@ int batch_step_n(vpfp_batch_s* b, void* f, double* e, int n, double* series, hip
       // first half kick of this step (not fused: the closure ran in between)
       ra.e = e_cur;
       ra.flags = vp::RK_KICK_A;
-      if (int rc = row_dispatch<T>(ra, s, b->row_q)) return rc;
+      BPROF(KK_ROW, row_dispatch<T>(ra, s, b->row_q));
     }
     double* ei = b->ering + (size_t)(i & 1) * b->n * b->nx;
     vp::DriftArgs<T> da;
@@ -696,20 +797,23 @@ int batch_step_n(vpfp_batch_s* b, void* f, double* e, int n, double* series, hip
     fa.e_out = ei;
     fa.series = series ? series + (size_t)i * b->n * 2 : nullptr;
     fa.series_stride = 2;
-    if (int rc = drift_dispatch<T>(da, b->C, b->gx, s, b->n)) return rc;
-    if (int rc = field_dispatch(fa, true, s, b->n)) return rc;
+    BPROF(KK_DRIFT, drift_dispatch<T>(da, b->C, b->gx, s, b->n));
+    BPROF(KK_FIELD, field_dispatch(fa, true, s, b->n));
     e_cur = ei;
     ra.e = e_cur;
     ra.flags = vp::RK_KICK_A | vp::RK_COLLIDE | (i + 1 < n && !closure ? vp::RK_KICK_B : 0);
-    if (int rc = row_dispatch<T>(ra, s, b->row_q)) return rc;
+    BPROF(KK_ROW, row_dispatch<T>(ra, s, b->row_q));
     if (closure) {
+      if (int rc = prof_mark(pr, KK_OTHER, s)) return rc;
       if (int rc = vpfc_apply(b->closure, reinterpret_cast<float*>(f), b->n * b->nx, s)) {
         return fail(rc, std::string("closure: ") + vpfc_last_error());
       }
+      if (int rc = prof_mark(pr, KK_OTHER, s)) return rc;
     }
   }
+#undef BPROF
   if (n > 0) VPFP_HIP(hipMemcpyAsync(e, e_cur, sizeof(double) * b->n * b->nx, hipMemcpyDeviceToDevice, s));
-  return VPFP_OK;
+  return prof_collect(pr);
 }
 
 // ---------------------------------------------------------------- autotuner
@@ -848,14 +952,14 @@ int autotune_plan(vpfp_plan_t p, int n_steps, hipStream_t s) {
       }
     }
   const Shape keep{p->row_q, p->C, p->fuse, p->xcd, p->dtiles};
-  const bool prof = p->profiling;
-  p->profiling = false;
+  const bool prof = p->prof.on;
+  p->prof.on = false;
   double us = 0.0;
   const int rc = tune_search(
       cands, keep, [&](const Shape& c) { plan_shape(p, c); },
       [&](int n) { return do_step_n<T>(p, w.f, w.e, n, nullptr, nullptr, nullptr, nullptr, s); }, w, n_steps, s,
       &us);
-  p->profiling = prof;
+  p->prof.on = prof;
   if (rc) return rc;
   p->tuned_us = us;
   return VPFP_OK;
@@ -877,10 +981,13 @@ int autotune_batch(vpfp_batch_s* b, int n_steps, hipStream_t s) {
         }
     }
   const Shape keep{b->row_q, b->C, 0, b->xcd, b->dtiles};
+  const bool prof = b->prof.on;
+  b->prof.on = false;
   double us = 0.0;
   const int rc = tune_search(
       cands, keep, [&](const Shape& c) { batch_shape(b, c); },
       [&](int n) { return batch_step_n<T>(b, w.f, w.e, n, nullptr, s); }, w, n_steps, s, &us);
+  b->prof.on = prof;
   if (rc) return rc;
   b->tuned_us = us;
   return VPFP_OK;
@@ -897,6 +1004,278 @@ int check_tuning(const vpfp_tuning* t, int nv) {
   if (t->drift_tile > nv) return fail(VPFP_EBADARG, "drift_tile exceeds nv");
   if (t->fuse_field < -1 || t->fuse_field > 2) return fail(VPFP_EBADARG, "fuse_field must be -1, 0, 1 or 2");
   return VPFP_OK;
+}
+
+}  // namespace
+
+// ====================================================================== grouped scan step
+// Several batches (size classes of one scan) stepped together: one row
+// launch per distinct nv, one drift and one field launch per distinct nx,
+// each over the segments of the batches concerned.
+struct vpfp_group_s {
+  std::vector<vpfp_batch_s*> b;
+  int prec = VPFP_F64, delta = 0;
+  struct VGroup {
+    int nv;
+    std::vector<int> members;  // batch indices
+    int q = 0;                 // row values per thread (0: row-count rule)
+  };
+  struct XGroup {
+    int nx;
+    std::vector<int> members;
+    int C = 16;                // drift columns per tile
+    int per = 1;               // column tiles per drift workgroup
+  };
+  std::vector<VGroup> vg;
+  std::vector<XGroup> xg;
+  double tuned_us = 0.0;
+  Prof prof;
+  StreamLog streams;
+};
+
+namespace {
+
+int group_gx(const vpfp_group_s::XGroup& x, const vpfp_batch_s* b) { return drift_wgs(b->nv, x.C, x.per, true); }
+
+template <class T>
+vp::RowMulti<T> group_rows(const vpfp_group_s* g, const vpfp_group_s::VGroup& v, void* const* f,
+                           const double* const* e, int flags) {
+  vp::RowMulti<T> m;
+  std::memset(&m, 0, sizeof(m));
+  const vpfp_batch_s* b0 = g->b[v.members[0]];
+  vp::RowArgs<T>& a = m.base;
+  a.nv = b0->nv;
+  a.dk = b0->dk_v;
+  a.v0 = b0->v0;
+  a.dvv = b0->dvv;
+  a.dv = b0->dv;
+  a.tw = reinterpret_cast<const cx<T>*>(b0->tw_v);
+  a.op_dg = b0->op == VPFP_OP_DG;
+  a.dt_a = a.dt_b = 0.5 * b0->dt;
+  a.flags = flags;
+  if (g->delta) {
+    a.bg = b0->bg;
+    a.bg_hat = reinterpret_cast<const cx<T>*>(b0->bg_hat);
+  }
+  m.nseg = (int)v.members.size();
+  for (int i = 0; i < m.nseg; ++i) {
+    const int k = v.members[i];
+    const vpfp_batch_s* b = g->b[k];
+    m.seg[i].f = reinterpret_cast<T*>(f[k]);
+    m.seg[i].e = e[k];
+    m.seg[i].nu_dt = b->nu_dt;
+    m.seg[i].rows = b->n * b->nx;
+    m.seg[i].rows_per_inst = b->nx;
+  }
+  return m;
+}
+
+template <class T>
+vp::DriftMulti<T> group_drift(const vpfp_group_s* g, const vpfp_group_s::XGroup& x, void* const* f) {
+  vp::DriftMulti<T> m;
+  std::memset(&m, 0, sizeof(m));
+  const vpfp_batch_s* b0 = g->b[x.members[0]];
+  m.base.nx = b0->nx;
+  m.base.dt = b0->dt;
+  m.base.tw = reinterpret_cast<const cx<T>*>(b0->tw_x);
+  m.nseg = (int)x.members.size();
+  for (int i = 0; i < m.nseg; ++i) {
+    const int k = x.members[i];
+    const vpfp_batch_s* b = g->b[k];
+    m.seg[i].f = reinterpret_cast<T*>(f[k]);
+    m.seg[i].v = b->v;
+    m.seg[i].dk = b->dk_x;
+    m.seg[i].dens = b->dens;
+    m.seg[i].nv = b->nv;
+    m.seg[i].gx = group_gx(x, b);
+    m.seg[i].n_inst = b->n;
+  }
+  return m;
+}
+
+vp::FieldMulti group_field(const vpfp_group_s* g, const vpfp_group_s::XGroup& x, double* const* e_out,
+                           double* const* series) {
+  vp::FieldMulti m;
+  std::memset(&m, 0, sizeof(m));
+  const vpfp_batch_s* b0 = g->b[x.members[0]];
+  m.base.nx = b0->nx;
+  m.base.scale = b0->dv;
+  m.base.tw = b0->tw_xs;
+  m.base.series_stride = 2;
+  m.nseg = (int)x.members.size();
+  for (int i = 0; i < m.nseg; ++i) {
+    const int k = x.members[i];
+    const vpfp_batch_s* b = g->b[k];
+    m.seg[i].part = b->dens;
+    m.seg[i].scale = b->dv;
+    m.seg[i].ntiles = group_gx(x, b);
+    m.seg[i].okx = b->okx;
+    m.seg[i].e_out = e_out[k];
+    m.seg[i].series = series ? series[k] : nullptr;
+    m.seg[i].n_inst = b->n;
+  }
+  return m;
+}
+
+#define BPROF_G(kind, call)                               \
+  do {                                                    \
+    if (int rc_ = prof_mark(pr, kind, s)) return rc_;     \
+    if (int rc_ = (call)) return rc_;                     \
+    if (int rc_ = prof_mark(pr, kind, s)) return rc_;     \
+  } while (0)
+
+template <class T>
+int group_step_n(vpfp_group_s* g, void* const* f, double* const* e, int n, double* const* series, hipStream_t s) {
+  const size_t nb = g->b.size();
+  std::vector<const double*> e_cur(e, e + nb);
+  std::vector<double*> ei(nb), ser(nb);
+  Prof& pr = g->prof;
+  const int nvg = (int)g->vg.size(), nxg = (int)g->xg.size();
+  if (n > 0)
+    for (int j = 0; j < nvg; ++j)
+      BPROF_G(KK_ROW + 4 * j, row_multi_dispatch<T>(group_rows<T>(g, g->vg[j], f, e_cur.data(), vp::RK_KICK_A), s,
+                                                    g->vg[j].q));
+  for (int i = 0; i < n; ++i) {
+    for (size_t k = 0; k < nb; ++k) {
+      const vpfp_batch_s* b = g->b[k];
+      ei[k] = b->ering + (size_t)(i & 1) * b->n * b->nx;
+      ser[k] = series && series[k] ? series[k] + (size_t)i * b->n * 2 : nullptr;
+    }
+    for (int j = 0; j < nxg; ++j)
+      BPROF_G(KK_DRIFT + 4 * j, drift_multi_dispatch(group_drift<T>(g, g->xg[j], f), g->xg[j].C, s));
+    for (int j = 0; j < nxg; ++j)
+      BPROF_G(KK_FIELD + 4 * j, field_multi_dispatch(group_field(g, g->xg[j], ei.data(), ser.data()), s));
+    for (size_t k = 0; k < nb; ++k) e_cur[k] = ei[k];
+    const int flags = vp::RK_KICK_A | vp::RK_COLLIDE | (i + 1 < n ? vp::RK_KICK_B : 0);
+    for (int j = 0; j < nvg; ++j)
+      BPROF_G(KK_ROW + 4 * j, row_multi_dispatch<T>(group_rows<T>(g, g->vg[j], f, e_cur.data(), flags), s, g->vg[j].q));
+  }
+  if (n > 0)
+    for (size_t k = 0; k < nb; ++k) {
+      const vpfp_batch_s* b = g->b[k];
+      VPFP_HIP(hipMemcpyAsync(e[k], e_cur[k], sizeof(double) * b->n * b->nx, hipMemcpyDeviceToDevice, s));
+    }
+  return prof_collect(pr);
+}
+#undef BPROF_G
+
+// Launch shapes of a group: each row group's values per thread and each x
+// group's drift tile (columns, tiles per workgroup) timed on scratch state,
+// one kernel at a time, then the whole step with the chosen shapes.
+template <class T>
+int autotune_group(vpfp_group_s* g, int n_steps, hipStream_t s) {
+  const size_t nb = g->b.size();
+  struct Scratch {
+    std::vector<void*> f;
+    std::vector<double*> e;
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Scratch() {
+      for (void* p : f) (void)hipFree(p);
+      for (double* p : e) (void)hipFree(p);
+      if (a) (void)hipEventDestroy(a);
+      if (b) (void)hipEventDestroy(b);
+    }
+  } w;
+  for (size_t k = 0; k < nb; ++k) {
+    const vpfp_batch_s* b = g->b[k];
+    const size_t rows = (size_t)b->n * b->nx;
+    auto h = tune_state<T>(rows, b->nx, b->nv, b->v0, b->dvv);
+    if (g->delta)
+      for (size_t r = 0; r < rows; ++r)
+        for (int m = 0; m < b->nv; ++m) {
+          const double vm = b->v0 + m * b->dvv;
+          h[r * b->nv + m] = (T)((double)h[r * b->nv + m] - std::exp(-0.5 * vm * vm) / std::sqrt(6.283185307179586));
+        }
+    void* fp = nullptr;
+    double* ep = nullptr;
+    VPFP_HIP(hipMalloc(&fp, sizeof(T) * h.size()));
+    w.f.push_back(fp);
+    VPFP_HIP(hipMalloc((void**)&ep, sizeof(double) * rows));
+    w.e.push_back(ep);
+    VPFP_HIP(hipMemcpy(fp, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    VPFP_HIP(hipMemset(ep, 0, sizeof(double) * rows));
+  }
+  VPFP_HIP(hipEventCreate(&w.a));
+  VPFP_HIP(hipEventCreate(&w.b));
+  const bool prof = g->prof.on;
+  g->prof.on = false;
+  // microseconds per call of run() over n_steps calls (after one untimed call);
+  // VPFP_EUNSUPPORTED (a shape that does not fit) is passed through
+  auto time = [&](auto run, double* us) -> int {
+    if (int rc = run()) return rc;
+    VPFP_HIP(hipEventRecord(w.a, s));
+    for (int i = 0; i < n_steps; ++i)
+      if (int rc = run()) return rc;
+    VPFP_HIP(hipEventRecord(w.b, s));
+    VPFP_HIP(hipEventSynchronize(w.b));
+    float ms = 0.f;
+    VPFP_HIP(hipEventElapsedTime(&ms, w.a, w.b));
+    *us = 1e3 * (double)ms / n_steps;
+    return VPFP_OK;
+  };
+  int rc = VPFP_OK;
+  const int flags = vp::RK_KICK_A | vp::RK_COLLIDE | vp::RK_KICK_B;
+  for (auto& v : g->vg) {
+    double best = 1e300;
+    int bq = v.q;
+    for (int q : row_q_candidates(v.nv)) {
+      double us = 0.0;
+      const int r = time([&]() { return row_multi_dispatch<T>(group_rows<T>(g, v, w.f.data(), w.e.data(), flags), s, q); },
+                         &us);
+      if (r == VPFP_EUNSUPPORTED) continue;
+      if (r) {
+        rc = r;
+        break;
+      }
+      if (us < best) {
+        best = us;
+        bq = q;
+      }
+    }
+    if (rc) break;
+    v.q = bq;
+  }
+  std::vector<double*> eo(w.e.begin(), w.e.end());
+  for (auto& x : g->xg) {
+    if (rc) break;
+    double best = 1e300;
+    int bC = x.C, bper = x.per;
+    for (int C : {4, 8, 16, 32})
+      for (int per : {1, 2, 4}) {
+        bool fits = true;
+        for (int k : x.members)
+          if (C > g->b[k]->nv || (per > 1 && g->b[k]->nv / C / per < 2)) fits = false;
+        if (!fits) continue;
+        x.C = C;
+        x.per = per;
+        double us = 0.0;
+        const int r = time(
+            [&]() -> int {
+              if (int rr = drift_multi_dispatch(group_drift<T>(g, x, w.f.data()), x.C, s)) return rr;
+              return field_multi_dispatch(group_field(g, x, eo.data(), nullptr), s);
+            },
+            &us);
+        if (r == VPFP_EUNSUPPORTED) continue;
+        if (r) {
+          rc = r;
+          break;
+        }
+        if (us < best) {
+          best = us;
+          bC = C;
+          bper = per;
+        }
+      }
+    x.C = bC;
+    x.per = bper;
+  }
+  if (!rc) {
+    double us = 0.0;
+    rc = time([&]() { return group_step_n<T>(g, w.f.data(), eo.data(), 1, nullptr, s); }, &us);
+    if (!rc) g->tuned_us = us;
+  }
+  g->prof.on = prof;
+  return rc;
 }
 
 }  // namespace
@@ -1251,6 +1630,7 @@ int vpfp_batch_create(const vpfp_batch_desc* d, vpfp_batch_t* out) {
 
 int vpfp_batch_set_closure(vpfp_batch_t b, vpfc_net_t net) {
   if (!b) return fail(VPFP_EBADARG, "null batch");
+  if (b->groups) return fail(VPFP_EUNSUPPORTED, "the batch belongs to a group (grouped steps run no closure)");
   if (net && (b->prec != VPFP_F32 || b->nv != VPFC_NV))
     return fail(VPFP_EUNSUPPORTED, "the closure runs on FP32 batches with nv = " + std::to_string(VPFC_NV));
   if (net && b->delta) return fail(VPFP_EUNSUPPORTED, "the closure reads f itself: not with delta-f storage");
@@ -1261,6 +1641,7 @@ int vpfp_batch_set_closure(vpfp_batch_t b, vpfc_net_t net) {
 
 int vpfp_batch_set_delta(vpfp_batch_t b, int on) {
   if (!b) return fail(VPFP_EBADARG, "null batch");
+  if (b->groups && on != b->delta) return fail(VPFP_EUNSUPPORTED, "the batch belongs to a group: its storage is fixed");
   if (on != 0 && on != 1) return fail(VPFP_EBADARG, "delta must be 0 or 1");
   if (on && b->closure) return fail(VPFP_EUNSUPPORTED, "the closure reads f itself: not with delta-f storage");
   if (on)
@@ -1370,12 +1751,7 @@ int vpfp_batch_autotune(vpfp_batch_t b, int n_steps, void* stream, vpfp_tuning* 
 
 int vpfp_plan_set_profiling(vpfp_plan_t p, int on) {
   if (int rc = check_plan(p)) return rc;
-  p->profiling = on != 0;
-  for (int k = 0; k < 4; ++k) {
-    p->kernel_ms[k] = 0.0;
-    p->kernel_n[k] = 0;
-  }
-  p->ev_used = 0;
+  p->prof.reset(on != 0);
   return VPFP_OK;
 }
 
@@ -1383,8 +1759,24 @@ int vpfp_plan_kernel_times(vpfp_plan_t p, double* ms, long long* counts) {
   if (int rc = check_plan(p)) return rc;
   if (!ms || !counts) return fail(VPFP_EBADARG, "null array");
   for (int k = 0; k < 4; ++k) {
-    ms[k] = p->kernel_ms[k];
-    counts[k] = p->kernel_n[k];
+    ms[k] = p->prof.ms[k];
+    counts[k] = p->prof.n[k];
+  }
+  return VPFP_OK;
+}
+
+int vpfp_batch_set_profiling(vpfp_batch_t b, int on) {
+  if (!b) return fail(VPFP_EBADARG, "null batch");
+  b->prof.reset(on != 0);
+  return VPFP_OK;
+}
+
+int vpfp_batch_kernel_times(vpfp_batch_t b, double* ms, long long* counts) {
+  if (!b) return fail(VPFP_EBADARG, "null batch");
+  if (!ms || !counts) return fail(VPFP_EBADARG, "null array");
+  for (int k = 0; k < 4; ++k) {
+    ms[k] = b->prof.ms[k];
+    counts[k] = b->prof.n[k];
   }
   return VPFP_OK;
 }
@@ -1399,6 +1791,179 @@ int vpfp_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, 
   if (p->nx > 2048) return fail(VPFP_EUNSUPPORTED, "nx too large for the fused step");
   return p->prec == VPFP_F64 ? do_step_n<double>(p, f, e, n, driver, e_hist, moments, series, p->streams.use(stream))
                              : do_step_n<float>(p, f, e, n, driver, e_hist, moments, series, p->streams.use(stream));
+}
+
+int vpfp_group_create(vpfp_batch_t const* batches, int n_batches, vpfp_group_t* out) {
+  if (!batches || !out) return fail(VPFP_EBADARG, "null argument");
+  *out = nullptr;
+  if (n_batches < 1 || n_batches > 64) return fail(VPFP_EBADARG, "n_batches must be in [1, 64]");
+  auto* g = new vpfp_group_s();
+  for (int k = 0; k < n_batches; ++k) {
+    vpfp_batch_s* b = batches[k];
+    if (!b) {
+      delete g;
+      return fail(VPFP_EBADARG, "null batch");
+    }
+    const vpfp_batch_s* b0 = batches[0];
+    if (b->prec != b0->prec || b->op != b0->op || b->dt != b0->dt || b->delta != b0->delta) {
+      delete g;
+      return fail(VPFP_EUNSUPPORTED, "the batches of a group must share precision, operator, dt and delta-f storage");
+    }
+    if (b->closure) {
+      delete g;
+      return fail(VPFP_EUNSUPPORTED, "a batch with a closure cannot join a group");
+    }
+    g->b.push_back(b);
+  }
+  g->prec = batches[0]->prec;
+  g->delta = batches[0]->delta;
+  // groups in order of first appearance; defaults from the largest member
+  for (int k = 0; k < n_batches; ++k) {
+    const vpfp_batch_s* b = g->b[k];
+    auto v = std::find_if(g->vg.begin(), g->vg.end(), [&](const vpfp_group_s::VGroup& q) { return q.nv == b->nv; });
+    if (v == g->vg.end()) {
+      g->vg.push_back({b->nv, {}, b->row_q});
+      v = g->vg.end() - 1;
+    }
+    const vpfp_batch_s* m0 = v->members.empty() ? nullptr : g->b[v->members[0]];
+    if (m0 && (m0->v0 != b->v0 || m0->dvv != b->dvv || m0->dk_v != b->dk_v)) {
+      delete g;
+      return fail(VPFP_EUNSUPPORTED, "batches of one nv must share the v grid");
+    }
+    v->members.push_back(k);
+    auto x = std::find_if(g->xg.begin(), g->xg.end(), [&](const vpfp_group_s::XGroup& q) { return q.nx == b->nx; });
+    if (x == g->xg.end()) {
+      g->xg.push_back({b->nx, {}, b->C, b->dtiles});
+      x = g->xg.end() - 1;
+    }
+    x->members.push_back(k);
+  }
+  for (const auto& v : g->vg)
+    if (v.members.size() > (size_t)vp::kMaxSeg) {
+      delete g;
+      return fail(VPFP_EUNSUPPORTED, "at most " + std::to_string(vp::kMaxSeg) + " batches per nv in a group");
+    }
+  for (auto& x : g->xg) {
+    if (x.members.size() > (size_t)vp::kMaxSeg) {
+      delete g;
+      return fail(VPFP_EUNSUPPORTED, "at most " + std::to_string(vp::kMaxSeg) + " batches per nx in a group");
+    }
+    int minnv = 1 << 30;
+    for (int k : x.members) minnv = std::min(minnv, g->b[k]->nv);
+    while (x.C > minnv) x.C /= 2;
+  }
+  for (vpfp_batch_s* b : g->b) ++b->groups;
+  *out = g;
+  return VPFP_OK;
+}
+
+int vpfp_group_destroy(vpfp_group_t g) {
+  if (!g) return VPFP_OK;
+  g->streams.drain();
+  for (vpfp_batch_s* b : g->b) --b->groups;
+  delete g;
+  return VPFP_OK;
+}
+
+int vpfp_group_step_n(vpfp_group_t g, void* const* f, double* const* e, int n_steps, double* const* series,
+                      void* stream) {
+  if (!g) return fail(VPFP_EBADARG, "null group");
+  if (!f || !e) return fail(VPFP_EBADARG, "null array");
+  for (size_t k = 0; k < g->b.size(); ++k)
+    if (!f[k] || !e[k]) return fail(VPFP_EBADARG, "null array of batch " + std::to_string(k));
+  if (n_steps < 0) return fail(VPFP_EBADARG, "negative step count");
+  hipStream_t s = g->streams.use(stream);
+  return g->prec == VPFP_F64 ? group_step_n<double>(g, f, e, n_steps, series, s)
+                             : group_step_n<float>(g, f, e, n_steps, series, s);
+}
+
+int vpfp_group_autotune(vpfp_group_t g, int n_steps, void* stream, double* us_per_step) {
+  if (!g) return fail(VPFP_EBADARG, "null group");
+  if (n_steps < 1) return fail(VPFP_EBADARG, "n_steps must be positive");
+  hipStream_t s = g->streams.use(stream);
+  const int rc = g->prec == VPFP_F64 ? autotune_group<double>(g, n_steps, s) : autotune_group<float>(g, n_steps, s);
+  if (rc) return rc;
+  if (us_per_step) *us_per_step = g->tuned_us;
+  return VPFP_OK;
+}
+
+int vpfp_group_get_shapes(vpfp_group_t g, int* row_q, int* drift_tile, int* drift_tiles) {
+  if (!g) return fail(VPFP_EBADARG, "null group");
+  if (!row_q || !drift_tile || !drift_tiles) return fail(VPFP_EBADARG, "null array");
+  for (size_t k = 0; k < g->b.size(); ++k) {
+    for (const auto& v : g->vg)
+      if (std::find(v.members.begin(), v.members.end(), (int)k) != v.members.end()) row_q[k] = v.q;
+    for (const auto& x : g->xg)
+      if (std::find(x.members.begin(), x.members.end(), (int)k) != x.members.end()) {
+        drift_tile[k] = x.C;
+        drift_tiles[k] = x.per;
+      }
+  }
+  return VPFP_OK;
+}
+
+int vpfp_group_set_shapes(vpfp_group_t g, const int* row_q, const int* drift_tile, const int* drift_tiles) {
+  if (!g) return fail(VPFP_EBADARG, "null group");
+  if (!row_q || !drift_tile || !drift_tiles) return fail(VPFP_EBADARG, "null array");
+  for (const auto& v : g->vg) {
+    const int q = row_q[v.members[0]];
+    if (q != 0 && q != 4 && q != 8 && q != 16) return fail(VPFP_EBADARG, "row_q must be 0, 4, 8 or 16");
+  }
+  for (const auto& x : g->xg) {
+    const int C = drift_tile[x.members[0]], per = drift_tiles[x.members[0]];
+    if (C != 4 && C != 8 && C != 16 && C != 32) return fail(VPFP_EBADARG, "drift_tile must be 4, 8, 16 or 32");
+    if (per < 1 || per > 64) return fail(VPFP_EBADARG, "drift_tiles must be in [1, 64]");
+    for (int k : x.members)
+      if (C > g->b[k]->nv) return fail(VPFP_EBADARG, "drift_tile exceeds nv");
+  }
+  for (auto& v : g->vg) v.q = row_q[v.members[0]];
+  for (auto& x : g->xg) {
+    x.C = drift_tile[x.members[0]];
+    x.per = drift_tiles[x.members[0]];
+  }
+  return VPFP_OK;
+}
+
+int vpfp_group_launch_times(vpfp_group_t g, int kind, int index, double* ms, long long* count) {
+  if (!g) return fail(VPFP_EBADARG, "null group");
+  if (kind < 0 || kind > 3 || index < 0 || !ms || !count) return fail(VPFP_EBADARG, "bad argument");
+  const size_t id = (size_t)(kind + 4 * index);
+  *ms = id < g->prof.ms_id.size() ? g->prof.ms_id[id] : 0.0;
+  *count = id < g->prof.n_id.size() ? g->prof.n_id[id] : 0;
+  return VPFP_OK;
+}
+
+int vpfp_group_set_profiling(vpfp_group_t g, int on) {
+  if (!g) return fail(VPFP_EBADARG, "null group");
+  g->prof.reset(on != 0);
+  return VPFP_OK;
+}
+
+int vpfp_group_kernel_times(vpfp_group_t g, double* ms, long long* counts) {
+  if (!g) return fail(VPFP_EBADARG, "null group");
+  if (!ms || !counts) return fail(VPFP_EBADARG, "null array");
+  for (int k = 0; k < 4; ++k) {
+    ms[k] = g->prof.ms[k];
+    counts[k] = g->prof.n[k];
+  }
+  return VPFP_OK;
+}
+
+int vpfp_step_n_store(vpfp_plan_t p, void* f, double* e, int n, const double* driver, double* e_hist, double* moments,
+                      double* series, void* f_store, int n_modes, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (!f || !e || !f_store) return fail(VPFP_EBADARG, "null array");
+  if (n < 0) return fail(VPFP_EBADARG, "negative step count");
+  if (n_modes < 0 || n_modes > p->nx) return fail(VPFP_EBADARG, "n_modes must be in [0, nx]");
+  if (int rc = need_spectral_v(p)) return rc;
+  if (int rc = need_spectral_x(p)) return rc;
+  if (p->nx > 2048) return fail(VPFP_EUNSUPPORTED, "nx too large for the fused step");
+  if (n_modes > 0 && !p->fcopy) {
+    if (int rc = dalloc(p, &p->fcopy, (size_t)p->nx * p->nv * (p->prec == VPFP_F64 ? 8 : 4))) return rc;
+  }
+  return p->prec == VPFP_F64
+             ? do_step_n<double>(p, f, e, n, driver, e_hist, moments, series, p->streams.use(stream), f_store, n_modes)
+             : do_step_n<float>(p, f, e, n, driver, e_hist, moments, series, p->streams.use(stream), f_store, n_modes);
 }
 
 }  // extern "C"

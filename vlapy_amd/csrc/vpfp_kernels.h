@@ -280,6 +280,7 @@ struct RowArgs {
   double nu_dt;
   int op_dg;
   double* moments;  // [kMoments][nx]
+  T* f_copy;        // [nx][nv] or null: f after the collision and moments (f itself under delta-f storage)
   int flags;
   const double* nu_dt_inst;  // batched scan: nu*dt per instance (rows are instance-major), or null
   int rows_per_inst;         // x rows per instance in a batch
@@ -363,7 +364,21 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
   };
   if (a.flags & RK_KICK_A) kick(a.dt_a * e);
 
-  if (a.flags & (RK_COLLIDE | RK_MOMENTS)) {
+  const bool chunk = (a.flags & (RK_COLLIDE | RK_MOMENTS)) != 0;
+  if (a.f_copy && !chunk && valid) {
+    // copy only: natural order, one complex pair per value (coalesced)
+    cx<T>* dst = reinterpret_cast<cx<T>*>(a.f_copy + (size_t)row * a.nv);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      cx<T> z = v[i];
+      if constexpr (BG) {
+        const int j = t + i * TPT;
+        z = mk<T>((T)((double)z.re + a.bg[2 * j]), (T)((double)z.im + a.bg[2 * j + 1]));
+      }
+      dst[t + i * TPT] = z;
+    }
+  }
+  if (chunk) {
     // natural order -> contiguous chunk of L samples per thread
 #pragma unroll
     for (int i = 0; i < Q; ++i) buf[pidx(t + i * TPT)] = v[i];
@@ -422,6 +437,13 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
 #pragma unroll
         for (int q = 0; q < kMoments; ++q) a.moments[(size_t)q * a.nx + row] = a.dv * s[q];
       }
+    }
+    if (a.f_copy && valid) {
+      // f after the collision (the reference's per-step f store); d holds f
+      // itself here also under delta-f storage
+      T* dst = a.f_copy + (size_t)row * a.nv + m0;
+#pragma unroll
+      for (int i = 0; i < L; ++i) dst[i] = (T)d[i];
     }
     if constexpr (BG) {
 #pragma unroll
@@ -826,6 +848,74 @@ VP_HD void tri_body(Ctx& c, unsigned char* lds, const TriArgs<T>& a) {
 #pragma unroll
     for (int i = 0; i < L; ++i) xr[t * L + i] = (T)d[i];
   }
+}
+
+// ---------------------------------------------------------------------------
+// several batches in one launch (vpfp_group_*): each kernel takes a table of
+// segments, one per batch, and its workgroups are numbered across them;
+// segment s owns the workgroups [blk0, blk0 of segment s+1).  The wrapper
+// picks the workgroup's segment, rebases the arguments and runs the body
+// with the workgroup index local to the segment.
+// ---------------------------------------------------------------------------
+constexpr int kMaxSeg = 8;
+
+template <class T>
+struct RowSeg {
+  T* f;  // read and written in place
+  const double* e;
+  const double* nu_dt;  // per instance
+  int rows;             // x rows of the batch (instances * nx)
+  int rows_per_inst;
+  int blk0;
+};
+template <class T>
+struct RowMulti {
+  RowArgs<T> base;  // shared: flags, dt, v grid, tables (one nv)
+  int nseg;
+  RowSeg<T> seg[kMaxSeg];
+};
+
+template <class T>
+struct DriftSeg {
+  T* f;
+  const double* v;
+  const double* dk;  // per instance
+  double* dens;
+  int nv;
+  int gx;            // workgroups per instance
+  int n_inst;
+  int blk0;
+};
+template <class T>
+struct DriftMulti {
+  DriftArgs<T> base;  // shared: nx, dt, twiddles
+  int nseg;
+  DriftSeg<T> seg[kMaxSeg];
+};
+
+struct FieldSeg {
+  const double* part;
+  double scale;  // dv of the batch's v grid
+  int ntiles;
+  const double* okx;
+  double* e_out;
+  double* series;
+  int n_inst;
+  int blk0;
+};
+struct FieldMulti {
+  FieldArgs base;  // shared: nx, scale, twiddles
+  int nseg;
+  FieldSeg seg[kMaxSeg];
+};
+
+// segment of workgroup b (blk0 ascending, seg[0].blk0 == 0)
+template <class S>
+VP_HD int seg_of(const S* seg, int nseg, int b) {
+  int s = 0;
+  for (int i = 1; i < nseg; ++i)
+    if (b >= seg[i].blk0) s = i;
+  return s;
 }
 
 }  // namespace vp

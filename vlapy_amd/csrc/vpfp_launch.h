@@ -78,9 +78,19 @@ int row_dispatch_plain(const vp::RowArgs<float>& a, hipStream_t s, int q);
 int row_dispatch_bg(const vp::RowArgs<double>& a, hipStream_t s, int q);
 int row_dispatch_bg(const vp::RowArgs<float>& a, hipStream_t s, int q);
 
+// The same over the segments of several batches of one nv (vpfp_group_*);
+// the dispatch fills in the segments' first workgroups
+int row_multi_dispatch_plain(const vp::RowMulti<double>& m, hipStream_t s, int q);
+int row_multi_dispatch_plain(const vp::RowMulti<float>& m, hipStream_t s, int q);
+int row_multi_dispatch_bg(const vp::RowMulti<double>& m, hipStream_t s, int q);
+int row_multi_dispatch_bg(const vp::RowMulti<float>& m, hipStream_t s, int q);
+
 // Drift kernels (vpfp_drift.hip): C columns per tile, gx workgroups striding
 // over the column tiles; gy > 0: a batch of gy instances (grid y = instance)
 int drift_dispatch(const vp::DriftArgs<double>& a, int C, int gx, hipStream_t s, int gy);
 int drift_dispatch(const vp::DriftArgs<float>& a, int C, int gx, hipStream_t s, int gy);
+// several batches of one nx (vpfp_group_*): segment s has n_inst * gx workgroups
+int drift_multi_dispatch(const vp::DriftMulti<double>& m, int C, hipStream_t s);
+int drift_multi_dispatch(const vp::DriftMulti<float>& m, int C, hipStream_t s);
 
 }  // namespace vpfp_impl

@@ -103,6 +103,27 @@ def step_n(plan, f, e, n_steps, driver=None, e_hist=None, moments=None, series=N
     return f, e
 
 
+def step_n_store(plan, f, e, n_steps, f_store, n_modes=0, driver=None, e_hist=None, moments=None, series=None):
+    """step_n with the per-step f store (vpfp_step_n_store): n_modes == 0:
+    f_store (n, nx, nv) in the storage dtype receives f after each step;
+    n_modes > 0: f_store (n, n_modes, nv) complex128 the first n_modes
+    x-modes of f after each step."""
+    nx, nv = plan.nx, plan.nv
+    n = int(n_steps)
+    nm = int(n_modes)
+    drv = _check(driver, (n, nx), torch.float64, "driver") if driver is not None else None
+    eh = _check(e_hist, (n, nx), torch.float64, "e_hist") if e_hist is not None else None
+    mo = _check(moments, (n, _lib.NMOMENTS, nx), torch.float64, "moments") if moments is not None else None
+    se = _check(series, (n, 2), torch.float64, "series") if series is not None else None
+    if nm == 0:
+        st = _check(f_store, (n, nx, nv), f_dtype(plan), "f_store")
+    else:
+        st = _check(f_store, (n, nm, nv), torch.complex128, "f_store")
+    _lib.check(_lib.library().vpfp_step_n_store(plan.handle, _f(plan, f), _x(plan, e, "e"), n, drv, eh, mo, se, st,
+                                                nm, _stream()))
+    return f, e
+
+
 def moments(plan, f, out):
     """Trapz moments [8][nx]: n, j, T, q, fv4, vN, f^2, f log f."""
     _lib.check(_lib.library().vpfp_moments(plan.handle, _f(plan, f),
@@ -117,3 +138,4 @@ def x_modes(plan, f, n_modes, out):
                                            _check(out, (int(n_modes), plan.nv), torch.complex128, "modes"),
                                            _stream()))
     return out
+

@@ -6,9 +6,11 @@ reference's temp_storage schema (outer_loop.py:149-215) with host numpy
 arrays, so the reference's storage manager and diagnostics can consume it.
 
 With the leapfrog integrator, exponential pushers and the spectral field
-solve, a batch of steps runs as the fused device loop vpfp_step_n (three
-kernels per step, f resident in HBM); any other combination runs the
-per-operator closures of vlapy_amd.core step by step.
+solve, a batch of steps runs as one call of the fused device loop
+vpfp_step_n_store (three kernels per step, f resident in HBM, the per-step f
+store written by the row kernel; a fourth launch per step for the k-mode
+store); any other combination runs the per-operator closures of
+vlapy_amd.core step by step.
 """
 
 import numpy as np
@@ -148,12 +150,9 @@ class _FusedInnerLoop:
         else:
             nm = len(self.rules)
             stored = torch.empty((n, nm, self.nv), dtype=torch.complex128, device=dev)
-        for i in range(n):
-            ops.step_n(self.plan, f, e, 1, None if drv_d is None else drv_d[i:i + 1], e_hist[i:i + 1], mom[i:i + 1])
-            if space == "all":
-                stored[i].copy_(f)
-            else:
-                ops.x_modes(self.plan, f, stored.shape[1], stored[i])
+        # the whole batch in one fused call; the row kernel writes f (or its
+        # x-modes) after every step's collision into `stored`
+        ops.step_n_store(self.plan, f, e, n, stored, 0 if space == "all" else stored.shape[1], drv_d, e_hist, mom)
         # back to the host storage
         eh = e_hist.cpu().numpy()
         mo = mom.cpu().numpy()

@@ -82,6 +82,9 @@ class _Class:
             f0 = f0 - maxw[None, None, :]
         self.f = f0.to(_dev.f_dtype(precision)).contiguous()
         self.e = torch.zeros(self.n, self.nx, dtype=torch.float64, device=dev)
+        self.v = torch.as_tensor(v, device=dev)
+        self.okx = torch.as_tensor(okx, device=dev)
+        self.dx = torch.as_tensor(np.asarray([g[0][1] - g[0][0] for g in grids]), device=dev)
 
     def full_f(self):
         """f itself (a copy in the storage type), also under delta-f storage."""
@@ -111,6 +114,16 @@ class _Class:
         tu.drift_order, tu.drift_tiles = int(t["drift_order"]), int(t["drift_tiles"])
         _lib.check(_lib.library().vpfp_batch_set_tuning(self.handle, ctypes.byref(tu)))
 
+    def set_profiling(self, on):
+        _lib.check(_lib.library().vpfp_batch_set_profiling(self.handle, 1 if on else 0))
+
+    def kernel_times(self):
+        """{kind: (total_ms, launches)} for kinds row, drift, field, other."""
+        ms = (ctypes.c_double * 4)()
+        n = (ctypes.c_longlong * 4)()
+        _lib.check(_lib.library().vpfp_batch_kernel_times(self.handle, ms, n))
+        return {k: (ms[i], n[i]) for i, k in enumerate(("row", "drift", "field", "other"))}
+
     def step(self, n_steps, series, stream):
         _lib.check(_lib.library().vpfp_batch_step_n(
             self.handle, ctypes.c_void_p(self.f.data_ptr()), ctypes.c_void_p(self.e.data_ptr()), int(n_steps),
@@ -134,7 +147,7 @@ class Scan:
     deviation from the Maxwellian instead of to f.
     """
 
-    def __init__(self, params, vmax=6.4, dt=0.1, op="dg", precision="fp32", delta=False):
+    def __init__(self, params, vmax=6.4, dt=0.1, op="dg", precision="fp32", delta=False, grouped=True):
         self.params = {key: np.asarray(val) for key, val in params.items()}
         self.n = self.params["k"].size
         self.vmax, self.dt, self.precision = float(vmax), float(dt), precision
@@ -146,14 +159,66 @@ class Scan:
             self.classes.append(_Class(idx, self.params, vmax, dt, op, precision, dev, delta))
         self.streams = [torch.cuda.Stream() for _ in self.classes]
         self.cells = int(np.sum(self.params["nx"].astype(np.int64) * self.params["nv"].astype(np.int64)))
+        # grouped: every class in one step of 10 launches at most (one row
+        # launch per nv, one drift and one field launch per nx; vpfp_group_*)
+        self.group = None
+        if grouped:
+            nb = len(self.classes)
+            arr = (ctypes.c_void_p * nb)(*[c.handle.value for c in self.classes])
+            h = ctypes.c_void_p()
+            _lib.check(_lib.library().vpfp_group_create(arr, nb, ctypes.byref(h)))
+            self.group = h
+        self.nv_order = list(dict.fromkeys(c.nv for c in self.classes))
+        self.nx_order = list(dict.fromkeys(c.nx for c in self.classes))
+
+    def _ptrs(self, tensors):
+        return (ctypes.c_void_p * len(tensors))(*[t.data_ptr() if t is not None else None for t in tensors])
+
+    def _group_step(self, n_steps, series, stream):
+        lib = _lib.library()
+        ser = self._ptrs(series) if series is not None else None
+        _lib.check(lib.vpfp_group_step_n(self.group, self._ptrs([c.f for c in self.classes]),
+                                         self._ptrs([c.e for c in self.classes]), int(n_steps), ser,
+                                         ctypes.c_void_p(stream.cuda_stream)))
+
+    def shapes(self):
+        """Grouped scans: the group's launch shapes per class {(nx, nv): tuning}."""
+        nb = len(self.classes)
+        q, C, per = (ctypes.c_int * nb)(), (ctypes.c_int * nb)(), (ctypes.c_int * nb)()
+        _lib.check(_lib.library().vpfp_group_get_shapes(self.group, q, C, per))
+        return {(c.nx, c.nv): {"row_q": q[i], "drift_tile": C[i], "fuse_field": 0, "drift_order": 0,
+                               "drift_tiles": per[i], "us_per_step": 0.0} for i, c in enumerate(self.classes)}
 
     def autotune(self, n_steps=5):
-        """Pick each size class's launch shapes by timing them (scratch state;
-        the instances are not advanced).  Returns {(nx, nv): tuning}."""
+        """Pick the launch shapes by timing them (scratch state; the instances
+        are not advanced): per nv and per nx for a grouped scan, per size
+        class otherwise.  Returns {(nx, nv): tuning}."""
+        if self.group is not None:
+            us = ctypes.c_double()
+            _lib.check(_lib.library().vpfp_group_autotune(self.group, int(n_steps),
+                                                          ctypes.c_void_p(torch.cuda.current_stream().cuda_stream),
+                                                          ctypes.byref(us)))
+            out = self.shapes()
+            for t in out.values():
+                t["us_per_step"] = us.value
+            return out
         out = {}
         for c, s in zip(self.classes, self.streams):
             out[(c.nx, c.nv)] = c.autotune(n_steps, s)
         return out
+
+    def set_tuning(self, tuning):
+        """Launch shapes as returned by autotune(): {(nx, nv): tuning}."""
+        if self.group is not None:
+            nb = len(self.classes)
+            t = [tuning[(c.nx, c.nv)] for c in self.classes]
+            q = (ctypes.c_int * nb)(*[int(x["row_q"]) for x in t])
+            C = (ctypes.c_int * nb)(*[int(x["drift_tile"]) for x in t])
+            per = (ctypes.c_int * nb)(*[max(1, int(x["drift_tiles"])) for x in t])
+            _lib.check(_lib.library().vpfp_group_set_shapes(self.group, q, C, per))
+            return
+        for c in self.classes:
+            c.set_tuning(tuning[(c.nx, c.nv)])
 
     def capture(self, n_steps):
         """Record n_steps of every size class (each class's launches on its own
@@ -166,11 +231,14 @@ class Scan:
         cap = torch.cuda.Stream()
         cap.wait_stream(torch.cuda.current_stream())
         with torch.cuda.graph(g, stream=cap, capture_error_mode="relaxed"):
-            for c, s in zip(self.classes, self.streams):
-                s.wait_stream(cap)
-                c.step(n_steps, None, s)
-            for s in self.streams:
-                cap.wait_stream(s)
+            if self.group is not None:
+                self._group_step(n_steps, None, cap)
+            else:
+                for c, s in zip(self.classes, self.streams):
+                    s.wait_stream(cap)
+                    c.step(n_steps, None, s)
+                for s in self.streams:
+                    cap.wait_stream(s)
         torch.cuda.current_stream().wait_stream(cap)
         self._graph = (g, int(n_steps))
         return self._graph[1]
@@ -190,16 +258,78 @@ class Scan:
         series = [torch.zeros(n_steps, c.n, 2, dtype=torch.float64, device=dev) if record else None
                   for c in self.classes]
         cur = torch.cuda.current_stream()
-        for c, s, ser in zip(self.classes, self.streams, series):
-            s.wait_stream(cur)
-            c.step(n_steps, ser, s)
-        for s in self.streams:
-            cur.wait_stream(s)
+        if self.group is not None:
+            self._group_step(n_steps, series if record else None, cur)
+        else:
+            for c, s, ser in zip(self.classes, self.streams, series):
+                s.wait_stream(cur)
+                c.step(n_steps, ser, s)
+            for s in self.streams:
+                cur.wait_stream(s)
         if not record:
             return None
         out = np.zeros((n_steps, self.n))
         for c, ser in zip(self.classes, series):
             out[:, c.idx] = ser[:, :, 0].cpu().numpy()
+        return out
+
+    def set_profiling(self, on):
+        """Per-kernel HIP event timing of every launch (off by default)."""
+        if self.group is not None:
+            _lib.check(_lib.library().vpfp_group_set_profiling(self.group, 1 if on else 0))
+            return
+        for c in self.classes:
+            c.set_profiling(on)
+
+    def kernel_times(self):
+        """{launch: {kind: (total_ms, launches)}} since profiling was switched on.
+        Per size class (nx, nv) for a scan of separate classes; for a grouped
+        scan per row launch ("nv", nv) and per drift / field launch ("nx", nx)."""
+        if self.group is None:
+            return {(c.nx, c.nv): c.kernel_times() for c in self.classes}
+        lib = _lib.library()
+        out = {}
+        for kind, name, order, key in ((0, "row", self.nv_order, "nv"), (1, "drift", self.nx_order, "nx"),
+                                       (2, "field", self.nx_order, "nx")):
+            for j, size in enumerate(order):
+                ms, n = ctypes.c_double(), ctypes.c_longlong()
+                _lib.check(lib.vpfp_group_launch_times(self.group, kind, j, ctypes.byref(ms), ctypes.byref(n)))
+                out.setdefault((key, size), {})[name] = (ms.value, n.value)
+        return out
+
+    def launch_cells(self):
+        """Cells each launch of kernel_times() covers: {launch: cells}."""
+        if self.group is None:
+            return {(c.nx, c.nv): c.n * c.nx * c.nv for c in self.classes}
+        out = {}
+        for c in self.classes:
+            for key in (("nv", c.nv), ("nx", c.nx)):
+                out[key] = out.get(key, 0) + c.n * c.nx * c.nv
+        return out
+
+    def __del__(self):
+        g = getattr(self, "group", None)
+        if g is not None and g.value and _lib is not None and _lib._lib is not None:
+            _lib._lib.vpfp_group_destroy(g)
+            self.group = None
+
+    def energy(self, self_consistent=False):
+        """Total energy per instance (host): kinetic dx sum_x trapz_v(f v^2 / 2)
+        plus field dx sum_x E^2 / 2.  E is the field of the last step, or with
+        self_consistent the Poisson field of the present f (E = Re ifft(i/kx
+        fft(1 - n)), as at the start of a run, where the stored e is zero)."""
+        out = np.zeros(self.n)
+        for c in self.classes:
+            f = c.full_f().double()
+            w = f * (0.5 * c.v ** 2)[None, None, :]
+            kin = c.dv * (w.sum(-1) - 0.5 * (w[..., 0] + w[..., -1]))
+            if self_consistent:
+                n = c.dv * (f.sum(-1) - 0.5 * (f[..., 0] + f[..., -1]))
+                e = torch.fft.ifft(1j * c.okx * torch.fft.fft(1.0 - n, dim=-1), dim=-1).real
+            else:
+                e = c.e
+            tot = c.dx * (kin.sum(-1) + 0.5 * (e ** 2).sum(-1))
+            out[c.idx] = tot.cpu().numpy()
         return out
 
     def density(self):
