@@ -99,6 +99,25 @@ int vpfp_solve_for_field(vpfp_plan_t plan, const double* charge_density, double*
  * solve_total_electric_field, field.py:77-86.  driver may be NULL (zero). */
 int vpfp_field(vpfp_plan_t plan, const void* f, const double* driver, double* e_out, void* stream);
 
+/* Allocation-free variant of vpfp_field: E = driver + solve_for_field(
+ * compute_charges(f)) for f (nx, nv) with no plan and no device allocation.
+ * No counterpart in the reference; it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  one_over_kx [nx] (1/kx, 0 at DC), f
+ * (double for VPFP_F64, float for VPFP_F32), driver [nx] (or NULL) and
+ * e_out [nx] are device pointers on the current device; stream belongs to
+ * that device.  Its scratch (density and twiddle table) is one file-scope
+ * device array of VPFP_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends and shared by every call on that GPU: calls that overlap on one GPU
+ * race.  nx > VPFP_NOALLOC_MAX_NX is refused with VPFP_EBADARG.  Power-of-two
+ * nx >= 4 takes the FFT field kernel (one workgroup), any other nx the direct
+ * DFT, as vpfp_field does; the twiddles are formed on the device (sincospi),
+ * so results agree with vpfp_field to rounding, not bit for bit. */
+#define VPFP_NOALLOC_MAX_NX 4096
+#define VPFP_NOALLOC_SCRATCH_BYTES 98304
+int vpfp_field_noalloc(int nx, int nv, int precision, double dv, const double* one_over_kx, const void* f,
+                       const double* driver, double* e_out, void* stream);
+
 /* Collision diagonals a [nx][nv-1], b [nx][nv], c [nx][nv-1] of f.
  * Replaces make_philharmonic_arrays_for_matrix / make_dougherty_arrays_for_matrix,
  * collisions.py:44-81, 104-158 (plan op, nu, dt, dv). */

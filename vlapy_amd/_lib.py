@@ -17,6 +17,7 @@ VPFP_OK, VPFP_EBADARG, VPFP_EUNSUPPORTED, VPFP_EHIP = 0, 1, 2, 3
 VPFP_F64, VPFP_F32 = 0, 1
 VPFP_OP_LB, VPFP_OP_DG = 0, 1
 NMOMENTS = 8
+NOALLOC_MAX_NX = 4096  # VPFP_NOALLOC_MAX_NX
 MOMENT_NAMES = ("n", "j", "T", "q", "fv4", "vN", "f2", "flogf")
 
 _c_p = ctypes.c_void_p
@@ -81,6 +82,7 @@ SIGNATURES = {
     "vpfp_compute_charges": [_c_p, _c_p, _c_p, _c_p],
     "vpfp_solve_for_field": [_c_p, _c_p, _c_p, _c_p],
     "vpfp_field": [_c_p, _c_p, _c_p, _c_p, _c_p],
+    "vpfp_field_noalloc": [_c_i, _c_i, _c_i, _c_d, _c_p, _c_p, _c_p, _c_p, _c_p],
     "vpfp_collision_diagonals": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
     "vpfp_tridiag_solve": [_c_p, _c_p, _c_p, _c_p, _c_p, _c_p, _c_p],
     "vpfp_collide": [_c_p, _c_p, _c_p, _c_p],

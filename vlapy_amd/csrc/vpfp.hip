@@ -17,6 +17,7 @@
 #include <cmath>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -127,6 +128,20 @@ __global__ __launch_bounds__((vp::RowCfg<T, M, Q>::wg)) void k_tri(vp::TriArgs<T
   extern __shared__ __align__(16) unsigned char lds[];
   VPFP_CTX;
   vp::tri_body<T, M, Q>(c, lds, a);
+}
+
+// Twiddles exp(-2 pi i n / N) built on the device (vpfp_field_noalloc):
+// split = 0: the full table w[n], n < N; split = 1: [lo | hi] with
+// lo[j] = W_N^j (j < 64), hi[h] = W_N^{64 h} (h < N / 64), the layout of
+// split_twiddles for N > vp::kTwFull
+__global__ __launch_bounds__(256) void k_twiddles(cx<double>* w, int N, int split) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n_entries = split ? 64 + (N / 64 > 1 ? N / 64 : 1) : N;
+  if (i >= n_entries) return;
+  const int n = split ? (i < 64 ? i : 64 * (i - 64)) : i;
+  double sn, cs;
+  sincospi(2.0 * (double)n / (double)N, &sn, &cs);
+  w[i] = vp::mk<double>(cs, -sn);
 }
 
 // ------------------------------------------------------------------ launch helpers
@@ -1360,6 +1375,40 @@ int no_delta(vpfp_plan_s* p, const char* what) {
 
 }  // namespace
 
+// ====================================================================== allocation-free field solve
+namespace {
+
+// Scratch of vpfp_field_noalloc: the density n(x) (VPFP_NOALLOC_MAX_NX
+// doubles) then the twiddle table (VPFP_NOALLOC_MAX_NX complex doubles).
+// Each GPU holds one copy until the process ends and every call on that GPU
+// shares it.  Host code never passes `scratch` itself to a kernel: its
+// device address comes from hipGetSymbolAddress, once per device.
+constexpr size_t kNoAllocTwOffset = (size_t)VPFP_NOALLOC_MAX_NX * sizeof(double);
+static_assert(kNoAllocTwOffset + (size_t)VPFP_NOALLOC_MAX_NX * sizeof(cx<double>) <= VPFP_NOALLOC_SCRATCH_BYTES,
+              "scratch layout");
+__device__ __align__(256) unsigned char scratch[VPFP_NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch;  // device address of `scratch`, by device ordinal
+
+// address of `scratch` on the current device (looked up on the first call
+// on that device)
+int scratch_address(unsigned char** out) {
+  int dev = 0;
+  VPFP_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((size_t)dev >= g_scratch.size()) g_scratch.resize((size_t)dev + 1, nullptr);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    VPFP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = reinterpret_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch[dev];
+  return VPFP_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 int vpfp_version(void) { return 1; }
@@ -1502,6 +1551,51 @@ int vpfp_field(vpfp_plan_t p, const void* f, const double* driver, double* e_out
   const bool pow2 = p->nx_pow2 && p->nx >= 4 && p->nx <= 4096;
   auto fa = field_args(p, p->dens, 1, 1.0, driver, e_out, nullptr, nullptr, pow2);
   return field_dispatch(fa, pow2, p->streams.use(stream));
+}
+
+int vpfp_field_noalloc(int nx, int nv, int precision, double dv, const double* one_over_kx, const void* f,
+                       const double* driver, double* e_out, void* stream) {
+  if (nx < 2 || nv < 2) return fail(VPFP_EBADARG, "nx and nv must be at least 2");
+  if (nx > VPFP_NOALLOC_MAX_NX)
+    return fail(VPFP_EBADARG, "nx = " + std::to_string(nx) + " exceeds the scratch of vpfp_field_noalloc (nx <= " +
+                                  std::to_string(VPFP_NOALLOC_MAX_NX) + ")");
+  if (precision != VPFP_F64 && precision != VPFP_F32) return fail(VPFP_EBADARG, "unknown precision");
+  if (!(dv > 0.0)) return fail(VPFP_EBADARG, "dv must be positive");
+  if (!one_over_kx || !f || !e_out) return fail(VPFP_EBADARG, "null array");
+  unsigned char* base = nullptr;
+  if (int rc = scratch_address(&base)) return rc;
+  double* dens = reinterpret_cast<double*>(base);
+  cx<double>* tw = reinterpret_cast<cx<double>*>(base + kNoAllocTwOffset);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  // power-of-two nx: the FFT field kernel in one workgroup (tables as
+  // split_twiddles lays them out); any other nx: the direct DFT (full table)
+  const bool pow2 = is_pow2(nx) && nx >= 4;
+  const int split = (pow2 && !vp::tw_full(nx, (int)sizeof(cx<double>))) ? 1 : 0;
+  const int entries = split ? 64 + (nx / 64 > 1 ? nx / 64 : 1) : nx;
+  hipLaunchKernelGGL(k_twiddles, dim3((entries + 255) / 256), dim3(256), 0, s, tw, nx, split);
+  if (int rc = launched()) return rc;
+  // n(x) = dv * trapz row sums: every entry is written before it is read
+  const int rows = 4;
+  if (precision == VPFP_F64)
+    hipLaunchKernelGGL(k_density<double>, dim3((nx + rows - 1) / rows), dim3(64 * rows), 0, s,
+                       reinterpret_cast<const double*>(f), nx, nv, dv, dens);
+  else
+    hipLaunchKernelGGL(k_density<float>, dim3((nx + rows - 1) / rows), dim3(64 * rows), 0, s,
+                       reinterpret_cast<const float*>(f), nx, nv, dv, dens);
+  if (int rc = launched()) return rc;
+  vp::FieldArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.part = dens;
+  a.ntiles = 1;
+  a.nx = nx;
+  a.scale = 1.0;
+  a.okx = one_over_kx;
+  a.tw = tw;
+  a.driver = driver;
+  a.e_out = e_out;
+  // one workgroup (batched form with one instance): no cross-workgroup
+  // counter or reduction buffer
+  return field_dispatch(a, pow2, s, pow2 ? 1 : 0);
 }
 
 int vpfp_collision_diagonals(vpfp_plan_t p, const void* f, double* a, double* b, double* c, void* stream) {

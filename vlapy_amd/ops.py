@@ -139,3 +139,18 @@ def x_modes(plan, f, n_modes, out):
                                            _stream()))
     return out
 
+
+def field_noalloc(nx, nv, precision, dv, one_over_kx, f, driver, e_out):
+    """E = driver + field of f with no plan and no device allocation
+    (vpfp_field_noalloc): one_over_kx (nx,) float64, f (nx, nv) float64 or
+    float32 (precision "fp64" / "fp32"), driver (nx,) or None, e_out (nx,)
+    float64, all device tensors.  Shares one scratch array per GPU with every
+    other call: do not overlap calls on one GPU."""
+    nx, nv = int(nx), int(nv)
+    fdt = torch.float64 if precision == "fp64" else torch.float32
+    drv = _check(driver, (nx,), torch.float64, "driver_field") if driver is not None else None
+    _lib.check(_lib.library().vpfp_field_noalloc(
+        nx, nv, _lib.VPFP_F64 if precision == "fp64" else _lib.VPFP_F32, float(dv),
+        _check(one_over_kx, (nx,), torch.float64, "one_over_kx"), _check(f, (nx, nv), fdt, "f"), drv,
+        _check(e_out, (nx,), torch.float64, "e"), _stream()))
+    return e_out
