@@ -210,16 +210,17 @@ int vpfp_batch_destroy(vpfp_batch_t batch);
  * vpfp_group_autotune times the row shape of each nv and the drift tile of
  * each nx on scratch state it allocates and keeps the fastest;
  * vpfp_group_get_shapes reports them per batch (row values per thread, drift
- * columns per tile, tiles per drift workgroup). */
+ * columns per tile, tiles per drift workgroup, MFMA drift as in vpfp_tuning). */
 typedef struct vpfp_group_s* vpfp_group_t;
 int vpfp_group_create(vpfp_batch_t const* batches, int n_batches, vpfp_group_t* out);
 int vpfp_group_step_n(vpfp_group_t group, void* const* f, double* const* e, int n_steps, double* const* series,
                       void* stream);
 int vpfp_group_autotune(vpfp_group_t group, int n_steps, void* stream, double* us_per_step /* or NULL */);
-int vpfp_group_get_shapes(vpfp_group_t group, int* row_q, int* drift_tile, int* drift_tiles);
+int vpfp_group_get_shapes(vpfp_group_t group, int* row_q, int* drift_tile, int* drift_tiles, int* drift_dft);
 /* set the shapes: each nv's row shape and each nx's drift tile are taken
  * from the first batch (in creation order) of that nv / nx */
-int vpfp_group_set_shapes(vpfp_group_t group, const int* row_q, const int* drift_tile, const int* drift_tiles);
+int vpfp_group_set_shapes(vpfp_group_t group, const int* row_q, const int* drift_tile, const int* drift_tiles,
+                          const int* drift_dft);
 /* per-kernel timing as vpfp_plan_set_profiling; vpfp_group_launch_times
  * gives one launch of a step: kind 0 row (index = the nv's order of first
  * appearance), 1 drift, 2 field (index = the nx's order) */
@@ -269,6 +270,8 @@ typedef struct vpfp_tuning {
   int drift_order;
   int drift_tiles;
   double us_per_step;
+  int drift_dft;  /* 1: FP32 x-advection as dense real-DFT products on MFMA
+                     (nx 32 or 64, drift_tile 16 or 32, not with fuse_field 1) */
 } vpfp_tuning;
 
 int vpfp_plan_get_tuning(vpfp_plan_t plan, vpfp_tuning* out);

@@ -167,8 +167,9 @@ def test_scan_delta_f_refuses_closure_and_round_trips_background():
     assert float((f - c64.f).abs().max()) < 1e-15
 
 
-@pytest.mark.parametrize("precision,delta", [("fp64", False), ("fp32", False), ("fp32", True)])
-def test_grouped_step_equals_per_class_batches(precision, delta):
+@pytest.mark.parametrize("precision,delta,dft", [("fp64", False, 0), ("fp32", False, 0), ("fp32", True, 0),
+                                               ("fp32", True, 1)])
+def test_grouped_step_equals_per_class_batches(precision, delta, dft):
     """The grouped step (one row launch per nv, one drift and one field launch
     per nx, vpfp_group_step_n) equals the per-class batches (vpfp_batch_step_n)
     bit for bit when both use the same launch shapes: the kernels run the
@@ -179,6 +180,11 @@ def test_grouped_step_equals_per_class_batches(precision, delta):
 
     p = _params()
     shape = {"row_q": 8, "drift_tile": 16, "fuse_field": 0, "drift_order": 0, "drift_tiles": 2, "us_per_step": 0.0}
+    if dft:
+        # the MFMA drift exists for nx 32 and 64 only: the nx = 128 class keeps
+        # the Stockham drift
+        p = {key: val[p["nx"] < 128] for key, val in p.items()}
+        shape["drift_dft"] = 1
     runs = []
     for grouped in (True, False):
         sc = Scan(p, dt=0.1, op="dg", precision=precision, delta=delta, grouped=grouped)
@@ -211,3 +217,29 @@ def test_grouped_scan_launches_per_step():
     launches = sum(v[1] for d in kt.values() for v in d.values())
     # 5 steps: one opening half-kick row launch per nv, then 10 per step
     assert launches == 4 + 5 * 10, kt
+
+
+def test_scan_mfma_drift_matches_oracle():
+    """A grouped FP32 delta-f scan with the MFMA drift on its nx = 32 and 64
+    classes matches the oracle within the FP32 scan's bounds."""
+    from vlapy_amd.scan import Scan
+
+    p = _params()
+    n, dt = 20, 0.1
+    sc = Scan(p, dt=dt, op="dg", precision="fp32", delta=True)
+    tun = sc.shapes()
+    for (nx, nv), t in tun.items():
+        if nx in (32, 64):
+            t.update(drift_tile=16, drift_dft=1)
+    sc.set_tuning(tun)
+    assert all(t["drift_dft"] == (1 if k[0] in (32, 64) else 0) for k, t in sc.shapes().items())
+    e2 = sc.run(n)
+    for i in range(p["k"].size):
+        f_o, e2_o = _oracle(p, i, n, dt)
+        err = np.abs(e2[:, i] - e2_o).max() / np.abs(e2_o).max()
+        print("mfma drift scan instance", i, "E^2 rel err", err)
+        assert err < 2e-3, (i, err)
+        c = [c for c in sc.classes if i in c.idx][0]
+        j = int(np.nonzero(c.idx == i)[0][0])
+        f_g = c.full_f()[j].double().cpu().numpy()
+        assert np.abs(f_g - f_o).max() / np.abs(f_o).max() < 2e-5, i

@@ -117,3 +117,50 @@ def test_batch_autotune_keeps_parity(grouped):
         _, _, eh = O.run_steps(gi, np.zeros(x.size), f0, n, 0.1, nu=p["nu"][i], fp_type="dg")
         ref = np.mean(eh ** 2, axis=1)
         assert np.abs(e2[:, i] - ref).max() / np.abs(ref).max() < 1e-11, i
+
+
+@pytest.mark.parametrize("nx,nv,C", [(32, 256, 16), (32, 512, 32), (64, 256, 16), (64, 1024, 32)])
+def test_mfma_drift_matches_oracle(nx, nv, C):
+    """The FP32 x-advection as dense real-DFT products on MFMA (drift_dft):
+    one vdfdx call within the FP32 bound of the Stockham drift (2e-6 of
+    max f), and 20 fused steps with the collision and field within the FP32
+    path's bound of the oracle."""
+    import torch
+
+    from vlapy_amd import ops
+    from vlapy_amd._lib import Plan
+
+    k, dt = 0.35, 0.1
+    print("mfma drift", nx, nv, C, flush=True)
+    g = O.Grid.make(nx, nv, 2 * np.pi / k, 6.4)
+    f0 = O.perturbed_maxwellian(g.x, g.v, k, 0.1)
+    p = Plan(nx, nv, "fp32", g.kx, g.one_over_kx, g.v, g.kv, g.dv, 1e-3, dt, "dg")
+    p.set_tuning(drift_tile=C, drift_dft=1, fuse_field=0)
+    assert p.tuning()["drift_dft"] == 1
+    f = torch.as_tensor(f0, dtype=torch.float32, device="cuda").contiguous()
+    out = ops.vdfdx(p, f, torch.empty_like(f), dt)
+    ref = O.vdfdx(f0, dt, g.kx, g.v)
+    err = rel(out.double().cpu().numpy(), ref)
+    print("  one call: %.2e" % err, flush=True)
+    assert err < 2e-6
+    n = 20
+    e = torch.zeros(nx, dtype=torch.float64, device="cuda")
+    eh = torch.zeros(n, nx, dtype=torch.float64, device="cuda")
+    ops.step_n(p, f, e, n, None, eh)
+    _, f_o, eh_o = O.run_steps(g, np.zeros(nx), f0, n, dt, nu=1e-3, fp_type="dg")
+    ef, ee = rel(f.double().cpu().numpy(), f_o), rel(eh.cpu().numpy(), eh_o)
+    print("  20 steps: f %.2e E %.2e" % (ef, ee), flush=True)
+    assert ef < 2e-5 and ee < 2e-3
+
+
+def test_mfma_drift_refuses_unsupported_plans():
+    from vlapy_amd._lib import Plan
+
+    g = O.Grid.make(128, 256, 2 * np.pi / 0.3, 6.4)
+    p = Plan(128, 256, "fp32", g.kx, g.one_over_kx, g.v, g.kv, g.dv, 0.0, 0.1, "lb")
+    with pytest.raises(NotImplementedError):
+        p.set_tuning(drift_dft=1)
+    g = O.Grid.make(32, 256, 2 * np.pi / 0.3, 6.4)
+    p = Plan(32, 256, "fp64", g.kx, g.one_over_kx, g.v, g.kv, g.dv, 0.0, 0.1, "lb")
+    with pytest.raises(NotImplementedError):
+        p.set_tuning(drift_dft=1)

@@ -66,11 +66,13 @@ class Tuning(ctypes.Structure):
         ("drift_order", _c_i),
         ("drift_tiles", _c_i),
         ("us_per_step", _c_d),
+        ("drift_dft", _c_i),
     ]
 
     def as_dict(self):
         return {"row_q": self.row_q, "drift_tile": self.drift_tile, "fuse_field": self.fuse_field,
-                "drift_order": self.drift_order, "drift_tiles": self.drift_tiles, "us_per_step": self.us_per_step}
+                "drift_order": self.drift_order, "drift_tiles": self.drift_tiles, "us_per_step": self.us_per_step,
+                "drift_dft": self.drift_dft}
 
 
 # name -> (argtypes); every function returns int status unless listed in _OTHER
@@ -103,8 +105,10 @@ SIGNATURES = {
     "vpfp_group_create": [ctypes.POINTER(_c_p), _c_i, ctypes.POINTER(_c_p)],
     "vpfp_group_step_n": [_c_p, ctypes.POINTER(_c_p), ctypes.POINTER(_c_p), _c_i, ctypes.POINTER(_c_p), _c_p],
     "vpfp_group_autotune": [_c_p, _c_i, _c_p, ctypes.POINTER(_c_d)],
-    "vpfp_group_get_shapes": [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)],
-    "vpfp_group_set_shapes": [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_i)],
+    "vpfp_group_get_shapes": [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_i),
+                              ctypes.POINTER(_c_i)],
+    "vpfp_group_set_shapes": [_c_p, ctypes.POINTER(_c_i), ctypes.POINTER(_c_i), ctypes.POINTER(_c_i),
+                              ctypes.POINTER(_c_i)],
     "vpfp_group_launch_times": [_c_p, _c_i, _c_i, ctypes.POINTER(_c_d), ctypes.POINTER(ctypes.c_longlong)],
     "vpfp_group_set_profiling": [_c_p, _c_i],
     "vpfp_group_kernel_times": [_c_p, ctypes.POINTER(_c_d), ctypes.POINTER(ctypes.c_longlong)],
@@ -241,7 +245,8 @@ class Plan:
         check(library().vpfp_plan_get_tuning(self.handle, ctypes.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, row_q=None, drift_tile=None, fuse_field=None, drift_order=None, drift_tiles=None):
+    def set_tuning(self, row_q=None, drift_tile=None, fuse_field=None, drift_order=None, drift_tiles=None,
+                   drift_dft=None):
         t = Tuning()
         check(library().vpfp_plan_get_tuning(self.handle, ctypes.byref(t)))
         if row_q is not None:
@@ -254,6 +259,8 @@ class Plan:
             t.drift_order = int(drift_order)
         if drift_tiles is not None:
             t.drift_tiles = int(drift_tiles)
+        if drift_dft is not None:
+            t.drift_dft = int(drift_dft)
         check(library().vpfp_plan_set_tuning(self.handle, ctypes.byref(t)))
 
     def autotune(self, n_steps=20, stream=None):

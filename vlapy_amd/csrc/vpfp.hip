@@ -342,6 +342,8 @@ struct vpfp_plan_s {
   int fuse = -1;   // field solved by the last drift workgroup: 1 yes, 0 no, -1 size rule
   int xcd = 1;     // drift tile order (xcd_block); never slower in the measured shapes
   int dtiles = 1;  // column tiles per drift workgroup
+  int dft = 0;                // x-advection as real-DFT products on MFMA (FP32, nx 32 or 64)
+  float* dft_mat = nullptr;   // [2][nx][nx] forward and inverse real-DFT matrices
   double tuned_us = 0.0;
   int delta = 0;                 // f stored as f - M(v)
   double* bg = nullptr;          // M(v_m) [nv] (FP64) and its DFT [nv] (storage precision)
@@ -409,6 +411,34 @@ std::vector<cx<T>> split_twiddles(int N) {
   }
   return w;
 }
+
+// Real-DFT matrices of the MFMA drift, [2][N][N] FP32 (rounded from long
+// double): forward F[k][x] = cos(2 pi k x / N) for k <= N/2 and
+// F[N/2 + j][x] = -sin(2 pi j x / N) for 0 < j < N/2 (rows Re F_k, Im F_k);
+// inverse G[x][k] = c_k cos(2 pi k x / N) / N for k <= N/2 (c = 1 at 0 and
+// N/2, else 2) and G[x][N/2 + j] = -2 sin(2 pi j x / N) / N, the real
+// synthesis from those rows.
+std::vector<float> dft_matrices(int N) {
+  std::vector<float> m(2 * (size_t)N * N);
+  const long double tp = 6.283185307179586476925286766559L;
+  auto ang = [&](long long k, int x) { return tp * (long double)((k * x) % N) / (long double)N; };
+  for (int row = 0; row < N; ++row)
+    for (int x = 0; x < N; ++x)
+      m[(size_t)row * N + x] = (float)(row <= N / 2 ? cosl(ang(row, x)) : -sinl(ang(row - N / 2, x)));
+  float* G = m.data() + (size_t)N * N;
+  for (int x = 0; x < N; ++x)
+    for (int col = 0; col < N; ++col) {
+      long double v;
+      if (col <= N / 2)
+        v = ((col == 0 || col == N / 2) ? 1.0L : 2.0L) * cosl(ang(col, x)) / N;
+      else
+        v = -2.0L * sinl(ang(col - N / 2, x)) / N;
+      G[(size_t)x * N + col] = (float)v;
+    }
+  return m;
+}
+
+bool dft_size(int prec, int nx) { return prec == VPFP_F32 && (nx == 32 || nx == 64); }
 
 template <class T, class P>
 int put(vpfp_plan_s* p, P** dst, const std::vector<T>& src) {
@@ -530,6 +560,7 @@ vp::DriftArgs<T> drift_args(vpfp_plan_t p, const void* fin, void* fout, double d
   std::memset(&a.fld, 0, sizeof(a.fld));
   a.dk_inst = nullptr;
   a.xcd_order = p->xcd;
+  a.dft = p->dft ? p->dft_mat : nullptr;
   return a;
 }
 
@@ -693,6 +724,8 @@ struct vpfp_batch_s {
   int row_q = 0;
   int xcd = 1;
   int dtiles = 1;
+  int dft = 0;
+  float* dft_mat = nullptr;
   double tuned_us = 0.0;
   vpfc_net_t closure = nullptr;  // learned closure after every step (FP32, nv 256)
   int groups = 0;                // groups holding this batch (vpfp_group_create)
@@ -797,6 +830,7 @@ int batch_step_n(vpfp_batch_s* b, void* f, double* e, int n, double* series, hip
     da.dens = b->dens;
     da.dk_inst = b->dk_x;
     da.xcd_order = b->xcd;
+    da.dft = b->dft ? b->dft_mat : nullptr;
     // the field is a launch of its own: a last-workgroup hand-off inside the
     // drift would need a device-scope fence (an L2 write-back and invalidate)
     // in each of the n * gx drift workgroups
@@ -898,7 +932,7 @@ std::vector<int> row_q_candidates(int nv) {
 
 // the launch shape under test / kept
 struct Shape {
-  int q, C, fuse, xcd, dt;
+  int q, C, fuse, xcd, dt, dft;
 };
 
 // Times every candidate shape with run(n) after `apply(shape)`; keeps the
@@ -931,6 +965,7 @@ int tune_search(const std::vector<Shape>& cands, const Shape& keep, Apply apply,
 }
 
 void plan_shape(vpfp_plan_t p, const Shape& c) {
+  p->dft = c.dft;
   p->row_q = c.q;
   p->C = c.C;
   p->fuse = c.fuse;
@@ -940,6 +975,7 @@ void plan_shape(vpfp_plan_t p, const Shape& c) {
 }
 
 void batch_shape(vpfp_batch_s* b, const Shape& c) {
+  b->dft = c.dft;
   b->row_q = c.q;
   b->C = c.C;
   b->xcd = c.xcd;
@@ -962,11 +998,13 @@ int autotune_plan(vpfp_plan_t p, int n_steps, hipStream_t s) {
           for (int dt : {1, 2, 4}) {
             // more tiles per workgroup only where it changes the grid
             if (dt > 1 && p->nv / C / dt < 8) continue;
-            cands.push_back({q, C, fuse, xo, dt});
+            cands.push_back({q, C, fuse, xo, dt, 0});
+            // the MFMA drift (no drift tail)
+            if (p->dft_mat && (C == 16 || C == 32) && fuse != 1) cands.push_back({q, C, fuse, xo, dt, 1});
           }
       }
     }
-  const Shape keep{p->row_q, p->C, p->fuse, p->xcd, p->dtiles};
+  const Shape keep{p->row_q, p->C, p->fuse, p->xcd, p->dtiles, p->dft};
   const bool prof = p->prof.on;
   p->prof.on = false;
   double us = 0.0;
@@ -992,10 +1030,11 @@ int autotune_batch(vpfp_batch_s* b, int n_steps, hipStream_t s) {
       for (int xo : {0, 1})
         for (int dt : {1, 2, 4}) {
           if (dt > 1 && b->nv / C / dt < 2) continue;
-          cands.push_back({q, C, 0, xo, dt});
+          cands.push_back({q, C, 0, xo, dt, 0});
+          if (b->dft_mat && (C == 16 || C == 32)) cands.push_back({q, C, 0, xo, dt, 1});
         }
     }
-  const Shape keep{b->row_q, b->C, 0, b->xcd, b->dtiles};
+  const Shape keep{b->row_q, b->C, 0, b->xcd, b->dtiles, b->dft};
   const bool prof = b->prof.on;
   b->prof.on = false;
   double us = 0.0;
@@ -1018,6 +1057,7 @@ int check_tuning(const vpfp_tuning* t, int nv) {
   if (t->drift_order != 0 && t->drift_order != 1) return fail(VPFP_EBADARG, "drift_order must be 0 or 1");
   if (t->drift_tile > nv) return fail(VPFP_EBADARG, "drift_tile exceeds nv");
   if (t->fuse_field < -1 || t->fuse_field > 2) return fail(VPFP_EBADARG, "fuse_field must be -1, 0, 1 or 2");
+  if (t->drift_dft != 0 && t->drift_dft != 1) return fail(VPFP_EBADARG, "drift_dft must be 0 or 1");
   return VPFP_OK;
 }
 
@@ -1040,6 +1080,7 @@ struct vpfp_group_s {
     std::vector<int> members;
     int C = 16;                // drift columns per tile
     int per = 1;               // column tiles per drift workgroup
+    int dft = 0;               // MFMA drift (FP32, nx 32 or 64)
   };
   std::vector<VGroup> vg;
   std::vector<XGroup> xg;
@@ -1093,6 +1134,7 @@ vp::DriftMulti<T> group_drift(const vpfp_group_s* g, const vpfp_group_s::XGroup&
   m.base.nx = b0->nx;
   m.base.dt = b0->dt;
   m.base.tw = reinterpret_cast<const cx<T>*>(b0->tw_x);
+  m.base.dft = x.dft ? b0->dft_mat : nullptr;
   m.nseg = (int)x.members.size();
   for (int i = 0; i < m.nseg; ++i) {
     const int k = x.members[i];
@@ -1254,15 +1296,19 @@ int autotune_group(vpfp_group_s* g, int n_steps, hipStream_t s) {
   for (auto& x : g->xg) {
     if (rc) break;
     double best = 1e300;
-    int bC = x.C, bper = x.per;
+    int bC = x.C, bper = x.per, bdft = x.dft;
+    const bool can_dft = g->b[x.members[0]]->dft_mat != nullptr;
+    for (int dft : {0, 1})
     for (int C : {4, 8, 16, 32})
       for (int per : {1, 2, 4}) {
+        if (dft && (!can_dft || (C != 16 && C != 32))) continue;
         bool fits = true;
         for (int k : x.members)
           if (C > g->b[k]->nv || (per > 1 && g->b[k]->nv / C / per < 2)) fits = false;
         if (!fits) continue;
         x.C = C;
         x.per = per;
+        x.dft = dft;
         double us = 0.0;
         const int r = time(
             [&]() -> int {
@@ -1279,10 +1325,12 @@ int autotune_group(vpfp_group_s* g, int n_steps, hipStream_t s) {
           best = us;
           bC = C;
           bper = per;
+          bdft = dft;
         }
       }
     x.C = bC;
     x.per = bper;
+    x.dft = bdft;
   }
   if (!rc) {
     double us = 0.0;
@@ -1457,6 +1505,7 @@ int vpfp_plan_create(const vpfp_grid_desc* d, vpfp_plan_t* out) {
   if (!rc) rc = upload(p->v, d->v, nvb);
   if (!rc) rc = upload(p->kv, d->kv, nvb);
   if (!rc) rc = p->prec == VPFP_F64 ? make_tables<double>(p) : make_tables<float>(p);
+  if (!rc && dft_size(p->prec, p->nx)) rc = put(p, &p->dft_mat, dft_matrices(p->nx));
   // partial density rows for the narrowest drift tile any tuning may pick
   if (!rc) rc = dalloc(p, &p->dens, nxb * (size_t)(p->C ? drift_grid(p->nv / 4) : 1));
   if (!rc) rc = dalloc(p, &p->ering, 2 * nxb);
@@ -1706,6 +1755,7 @@ int vpfp_batch_create(const vpfp_batch_desc* d, vpfp_batch_t* out) {
   if (!rc) rc = bput(b, &b->v, vv);
   if (!rc) rc = bput(b, &b->okx, okx);
   if (!rc) rc = b->prec == VPFP_F64 ? batch_tables<double>(b) : batch_tables<float>(b);
+  if (!rc && dft_size(b->prec, b->nx)) rc = bput(b, &b->dft_mat, dft_matrices(b->nx));
   // partial density rows for the narrowest drift tile any tuning may pick
   if (!rc) rc = balloc(b, &b->dens, sizeof(double) * (size_t)b->n * (b->nv / 4) * b->nx);
   if (!rc) rc = balloc(b, &b->counters, sizeof(unsigned) * 16 * (size_t)b->n);
@@ -1785,12 +1835,15 @@ int vpfp_plan_get_tuning(vpfp_plan_t p, vpfp_tuning* t) {
   t->drift_order = p->xcd;
   t->drift_tiles = p->dtiles;
   t->us_per_step = p->tuned_us;
+  t->drift_dft = p->dft;
   return VPFP_OK;
 }
 
 int vpfp_plan_set_tuning(vpfp_plan_t p, const vpfp_tuning* t) {
   if (int rc = check_plan(p)) return rc;
   if (int rc = check_tuning(t, p->nv)) return rc;
+  if (t->drift_dft && !p->dft_mat) return fail(VPFP_EUNSUPPORTED, "the MFMA drift needs FP32 and nx 32 or 64");
+  p->dft = t->drift_dft;
   p->row_q = t->row_q;
   if (t->drift_tiles) p->dtiles = t->drift_tiles;
   if (p->C && t->drift_tile) p->C = t->drift_tile;
@@ -1820,12 +1873,15 @@ int vpfp_batch_get_tuning(vpfp_batch_t b, vpfp_tuning* t) {
   t->drift_order = b->xcd;
   t->drift_tiles = b->dtiles;
   t->us_per_step = b->tuned_us;
+  t->drift_dft = b->dft;
   return VPFP_OK;
 }
 
 int vpfp_batch_set_tuning(vpfp_batch_t b, const vpfp_tuning* t) {
   if (!b) return fail(VPFP_EBADARG, "null batch");
   if (int rc = check_tuning(t, b->nv)) return rc;
+  if (t->drift_dft && !b->dft_mat) return fail(VPFP_EUNSUPPORTED, "the MFMA drift needs FP32 and nx 32 or 64");
+  b->dft = t->drift_dft;
   b->row_q = t->row_q;
   if (t->drift_tiles) b->dtiles = t->drift_tiles;
   if (t->drift_tile) b->C = t->drift_tile;
@@ -1927,7 +1983,7 @@ int vpfp_group_create(vpfp_batch_t const* batches, int n_batches, vpfp_group_t* 
     v->members.push_back(k);
     auto x = std::find_if(g->xg.begin(), g->xg.end(), [&](const vpfp_group_s::XGroup& q) { return q.nx == b->nx; });
     if (x == g->xg.end()) {
-      g->xg.push_back({b->nx, {}, b->C, b->dtiles});
+      g->xg.push_back({b->nx, {}, b->C, b->dtiles, b->dft_mat ? b->dft : 0});
       x = g->xg.end() - 1;
     }
     x->members.push_back(k);
@@ -1981,9 +2037,9 @@ int vpfp_group_autotune(vpfp_group_t g, int n_steps, void* stream, double* us_pe
   return VPFP_OK;
 }
 
-int vpfp_group_get_shapes(vpfp_group_t g, int* row_q, int* drift_tile, int* drift_tiles) {
+int vpfp_group_get_shapes(vpfp_group_t g, int* row_q, int* drift_tile, int* drift_tiles, int* drift_dft) {
   if (!g) return fail(VPFP_EBADARG, "null group");
-  if (!row_q || !drift_tile || !drift_tiles) return fail(VPFP_EBADARG, "null array");
+  if (!row_q || !drift_tile || !drift_tiles || !drift_dft) return fail(VPFP_EBADARG, "null array");
   for (size_t k = 0; k < g->b.size(); ++k) {
     for (const auto& v : g->vg)
       if (std::find(v.members.begin(), v.members.end(), (int)k) != v.members.end()) row_q[k] = v.q;
@@ -1991,14 +2047,16 @@ int vpfp_group_get_shapes(vpfp_group_t g, int* row_q, int* drift_tile, int* drif
       if (std::find(x.members.begin(), x.members.end(), (int)k) != x.members.end()) {
         drift_tile[k] = x.C;
         drift_tiles[k] = x.per;
+        drift_dft[k] = x.dft;
       }
   }
   return VPFP_OK;
 }
 
-int vpfp_group_set_shapes(vpfp_group_t g, const int* row_q, const int* drift_tile, const int* drift_tiles) {
+int vpfp_group_set_shapes(vpfp_group_t g, const int* row_q, const int* drift_tile, const int* drift_tiles,
+                          const int* drift_dft) {
   if (!g) return fail(VPFP_EBADARG, "null group");
-  if (!row_q || !drift_tile || !drift_tiles) return fail(VPFP_EBADARG, "null array");
+  if (!row_q || !drift_tile || !drift_tiles || !drift_dft) return fail(VPFP_EBADARG, "null array");
   for (const auto& v : g->vg) {
     const int q = row_q[v.members[0]];
     if (q != 0 && q != 4 && q != 8 && q != 16) return fail(VPFP_EBADARG, "row_q must be 0, 4, 8 or 16");
@@ -2007,6 +2065,9 @@ int vpfp_group_set_shapes(vpfp_group_t g, const int* row_q, const int* drift_til
     const int C = drift_tile[x.members[0]], per = drift_tiles[x.members[0]];
     if (C != 4 && C != 8 && C != 16 && C != 32) return fail(VPFP_EBADARG, "drift_tile must be 4, 8, 16 or 32");
     if (per < 1 || per > 64) return fail(VPFP_EBADARG, "drift_tiles must be in [1, 64]");
+    const int dft = drift_dft[x.members[0]];
+    if (dft != 0 && dft != 1) return fail(VPFP_EBADARG, "drift_dft must be 0 or 1");
+    if (dft && !g->b[x.members[0]]->dft_mat) return fail(VPFP_EUNSUPPORTED, "the MFMA drift needs FP32 and nx 32 or 64");
     for (int k : x.members)
       if (C > g->b[k]->nv) return fail(VPFP_EBADARG, "drift_tile exceeds nv");
   }
@@ -2014,6 +2075,7 @@ int vpfp_group_set_shapes(vpfp_group_t g, const int* row_q, const int* drift_til
   for (auto& x : g->xg) {
     x.C = drift_tile[x.members[0]];
     x.per = drift_tiles[x.members[0]];
+    x.dft = drift_dft[x.members[0]];
   }
   return VPFP_OK;
 }

@@ -20,8 +20,9 @@
 //   * row_sum<>    : sum over the TPT threads that own one sequence
 //   * tridiag      : partition solver for one diagonally dominant
 //                    tridiagonal system held by TPT threads (chunk sweeps,
-//                    parallel cyclic reduction of the 2*TPT interface
-//                    unknowns in LDS, Thomas on the chunk interiors)
+//                    elimination of the even interface unknowns, parallel
+//                    cyclic reduction of the TPT odd ones in LDS, Thomas on
+//                    the chunk interiors)
 #pragma once
 
 #if defined(__HIP__)
@@ -87,6 +88,24 @@ VP_HD double rcp(double x) {
 #else
   return 1.0 / x;
 #endif
+}
+// The same with one Newton step: the estimate's ~23 correct bits doubled to
+// ~46 (relative error ~2^-46), for the FP64 elimination of FP32-stored f,
+// whose own rounding (2^-24) is ten orders of magnitude larger.
+VP_HD double rcp46(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double r = __builtin_amdgcn_rcp(x);
+  return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+#else
+  return 1.0 / x;
+#endif
+}
+template <bool FULL>
+VP_HD double rcpp(double x) {
+  if constexpr (FULL)
+    return rcp(x);
+  else
+    return rcp46(x);
 }
 
 // ---------------------------------------------------------------------------
@@ -448,14 +467,21 @@ VP_HD void row_sum(Ctx& c, double* vals, double* scratch) {
 //
 // Row m reads  l(m) x_{m-1} + b(m) x_m + u(m) x_{m+1} = d_m  with l(0) = 0 and
 // u(N-1) = 0.  Thread t owns rows m0 = t*L .. m0+L-1 with d in `d[L]`; on
-// return `d` holds the solution.  Coef provides l(m), u(m), b(m).
-// red: 8*TPT doubles of LDS for the interface system (one system per
-// 2*TPT interface rows).
+// return `d` holds the solution.  Coef::view<L>(m0, first, last) gives the
+// thread's coefficients l(i), u(i), b(i) of row m0 + i (zero l(0) on the
+// first thread and u(L-1) on the last).
+// red: 5*TPT doubles of LDS for the interface system (one row per thread
+// after the even interface unknowns are eliminated).
 // ---------------------------------------------------------------------------
-template <int TPT, int L, class Coef, class Ctx>
+// FULL: reciprocals to full double precision (FP64 f); otherwise to ~2^-46
+// (FP32 f).
+template <int TPT, int L, class Coef, class Ctx, bool FULL = true>
 VP_HD void tridiag(Ctx& c, double* d, const Coef& cf, int t, double* red) {
   static_assert(L >= 2, "chunk must hold at least two rows");
   const int m0 = t * L;
+  // this thread's rows m0 .. m0+L-1: coefficients by chunk index i (the
+  // first thread's l(0) and the last thread's u(L-1) are zero)
+  const auto cv = cf.template view<L>(m0, t == 0, t == TPT - 1);
   // Two independent sweeps, interleaved so that their dependency chains
   // overlap:
   //   down (i = 1 .. L-1), x_{m0} kept free:   x_i = p  + q  x_{m0}     - r  x_{i+1}
@@ -465,79 +491,89 @@ VP_HD void tridiag(Ctx& c, double* d, const Coef& cf, int t, double* red) {
   for (int s = 0; s < L - 1; ++s) {
     {
       const int i = 1 + s;
-      const int m = m0 + i;
-      const double lm = cf.l(m), um = cf.u(m), bm = cf.b(m);
+      const double lm = cv.l(i), um = cv.u(i), bm = cv.b(i);
       const double den = (i == 1) ? bm : bm - lm * r;
-      const double inv = rcp(den);
+      const double inv = rcpp<FULL>(den);
       p = (i == 1) ? d[i] * inv : (d[i] - lm * p) * inv;
       q = (i == 1) ? -lm * inv : -lm * q * inv;
       r = um * inv;
     }
     {
       const int i = L - 2 - s;
-      const int m = m0 + i;
-      const double lm = cf.l(m), um = cf.u(m), bm = cf.b(m);
+      const double lm = cv.l(i), um = cv.u(i), bm = cv.b(i);
       const double den = (i == L - 2) ? bm : bm - um * r2;
-      const double inv = rcp(den);
+      const double inv = rcpp<FULL>(den);
       p2 = (i == L - 2) ? d[i] * inv : (d[i] - um * p2) * inv;
       q2 = (i == L - 2) ? -um * inv : -um * q2 * inv;
       r2 = lm * inv;
     }
   }
-  // interface system: y_{2t} = x_{m0}, y_{2t+1} = x_{m0+L-1}
+  // interface unknowns y_{2t} = x_{m0}, y_{2t+1} = x_{m0+L-1}:
   //   row 2t  :  r2 y_{2t-1} + y_{2t} - q2 y_{2t+1} = p2
   //   row 2t+1: -q  y_{2t}   + y_{2t+1} + r  y_{2t+2} = p
-  constexpr int NR = 2 * TPT;
-  double ra[2] = {r2, -q}, rb[2] = {1.0, 1.0}, rc[2] = {-q2, r}, rd[2] = {p2, p};
+  // Eliminating the even unknowns (y_{2t} by row 2t, y_{2t+2} by row 2t+2 of
+  // the next thread) leaves one row per thread in z_t = y_{2t+1}:
+  //   A z_{t-1} + B z_t + C z_{t+1} = D,
+  //   A = q r2, B = 1 - q q2 - r r2', C = r q2', D = p + q p2 - r p2'
+  // (primes: thread t+1's values; r = 0 on the last thread, r2 = 0 on the
+  // first).  It is a Schur complement of the diagonally dominant system,
+  // hence diagonally dominant itself, and is solved by parallel cyclic
+  // reduction, one row per thread.
   double* A = red;
-  double* B = red + NR;
-  double* C = red + 2 * NR;
-  double* D = red + 3 * NR;
-  for (int s = 1; s < NR; s <<= 1) {
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int i = 2 * t + h;
-      A[i] = ra[h];
-      B[i] = rb[h];
-      C[i] = rc[h];
-      D[i] = rd[h];
-    }
+  double* B = red + TPT;
+  double* C = red + 2 * TPT;
+  double* D = red + 3 * TPT;
+  A[t] = p2;
+  B[t] = q2;
+  C[t] = r2;
+  c.sync();
+  const bool last = t == TPT - 1;
+  const double p2n = last ? 0.0 : A[t + 1], q2n = last ? 0.0 : B[t + 1], r2n = last ? 0.0 : C[t + 1];
+  c.sync();
+  double ra = q * r2, rb = 1.0 - q * q2 - r * r2n, rc = r * q2n, rd = p + q * p2 - r * p2n;
+  // each thread publishes 1/B of its row with it (one reciprocal per level)
+  double* BI = red + 4 * TPT;
+  for (int s = 1; s < TPT; s <<= 1) {
+    A[t] = ra;
+    BI[t] = rcpp<FULL>(rb);
+    C[t] = rc;
+    D[t] = rd;
     c.sync();
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int i = 2 * t + h;
-      const int im = i - s, ip = i + s;
-      double na = 0.0, nb = rb[h], nc = 0.0, nd = rd[h];
-      if (im >= 0) {
-        const double al = -ra[h] * rcp(B[im]);
-        na = al * A[im];
-        nb += al * C[im];
-        nd += al * D[im];
-      }
-      if (ip < NR) {
-        const double ga = -rc[h] * rcp(B[ip]);
-        nc = ga * C[ip];
-        nb += ga * A[ip];
-        nd += ga * D[ip];
-      }
-      ra[h] = na;
-      rb[h] = nb;
-      rc[h] = nc;
-      rd[h] = nd;
+    const int im = t - s, ip = t + s;
+    double na = 0.0, nb = rb, nc = 0.0, nd = rd;
+    if (im >= 0) {
+      const double al = -ra * BI[im];
+      na = al * A[im];
+      nb += al * C[im];
+      nd += al * D[im];
     }
+    if (ip < TPT) {
+      const double ga = -rc * BI[ip];
+      nc = ga * C[ip];
+      nb += ga * A[ip];
+      nd += ga * D[ip];
+    }
+    ra = na;
+    rb = nb;
+    rc = nc;
+    rd = nd;
     c.sync();
   }
-  const double xs = rd[0] * rcp(rb[0]);
-  const double xe = rd[1] * rcp(rb[1]);
+  const double xe = rd * rcpp<FULL>(rb);
+  // y_{2t} = p2 - r2 z_{t-1} + q2 z_t
+  A[t] = xe;
+  c.sync();
+  const double zm = t > 0 ? A[t - 1] : 0.0;
+  c.sync();
+  const double xs = p2 - r2 * zm + q2 * xe;
   // interior rows m0+1 .. m0+L-2 with both chunk ends known (Thomas)
   double rr[L];
   d[0] = xs;
 #pragma unroll
   for (int i = 1; i < L - 1; ++i) {
-    const int m = m0 + i;
-    const double lm = cf.l(m), um = cf.u(m), bm = cf.b(m);
+    const double lm = cv.l(i), um = cv.u(i), bm = cv.b(i);
     const double den = (i == 1) ? bm : bm - lm * rr[i - 1];
-    const double inv = rcp(den);
+    const double inv = rcpp<FULL>(den);
     d[i] = (i == 1) ? (d[i] - lm * xs) * inv : (d[i] - lm * d[i - 1]) * inv;
     rr[i] = um * inv;
   }

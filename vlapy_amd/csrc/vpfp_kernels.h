@@ -63,9 +63,20 @@ struct CollCoef {
     u0 = ndt * (A0 - ((v0 + dvv) - vbar) * H);
     bd = bd_;
   }
-  VP_HD double l(int m) const { return m == 0 ? 0.0 : l0 + ls * (double)m; }
-  VP_HD double u(int m) const { return m == N - 1 ? 0.0 : u0 - ls * (double)m; }
-  VP_HD double b(int) const { return bd; }
+  // rows m0 + i of one thread: l and u affine in the chunk index i (one FMA
+  // each), the ends of the whole row zero
+  template <int L>
+  struct View {
+    double lb, ub, ls, bd;
+    bool first, last;
+    VP_HD double l(int i) const { return (i == 0 && first) ? 0.0 : lb + ls * (double)i; }
+    VP_HD double u(int i) const { return (i == L - 1 && last) ? 0.0 : ub - ls * (double)i; }
+    VP_HD double b(int) const { return bd; }
+  };
+  template <int L>
+  VP_HD View<L> view(int m0, bool first, bool last) const {
+    return View<L>{l0 + ls * (double)m0, u0 - ls * (double)m0, ls, bd, first, last};
+  }
 };
 
 // explicit diagonals a (nv-1), b (nv), c (nv-1) of one row
@@ -74,29 +85,55 @@ struct ArrCoef {
   const double* bb;
   const double* cc;
   int N;
-  VP_HD double l(int m) const { return m == 0 ? 0.0 : a[m - 1]; }
-  VP_HD double u(int m) const { return m == N - 1 ? 0.0 : cc[m]; }
-  VP_HD double b(int m) const { return bb[m]; }
+  template <int L>
+  struct View {
+    const double *a, *bb, *cc;
+    int m0, N;
+    bool first, last;
+    // indices clamped into the arrays: the row ends are never read
+    VP_HD double l(int i) const {
+      const int k = m0 + i - 1;
+      return (i == 0 && first) ? 0.0 : a[k > 0 ? k : 0];
+    }
+    VP_HD double u(int i) const {
+      const int k = m0 + i;
+      return (i == L - 1 && last) ? 0.0 : cc[k < N - 2 ? k : N - 2];
+    }
+    VP_HD double b(int i) const { return bb[m0 + i]; }
+  };
+  template <int L>
+  VP_HD View<L> view(int m0, bool first, bool last) const {
+    return View<L>{a, bb, cc, m0, N, first, last};
+  }
 };
 
 // drift velocity and temperature of one row (un-normalised, trapz), chunk layout
+// trapz weight of chunk element i of thread t (rows t L .. t L + L - 1 of a
+// row of TPT L samples): 1/2 at the two ends of the row
+template <int TPT, int L>
+VP_HD double chunk_w(int i, int t) {
+  return ((i == 0 && t == 0) || (i == L - 1 && t == TPT - 1)) ? 0.5 : 1.0;
+}
+
 template <int TPT, int L, class Ctx>
 VP_HD void coll_moments(Ctx& c, const double* d, double v0, double dvv, int N, double dv, bool dg, int t,
                         double* rs, double& vbar, double& temp) {
   const int m0 = t * L;
+  // v_{m0+i} = vb + i dvv
+  const double vb = v0 + (double)m0 * dvv;
   double s[1] = {0.0};
   vbar = 0.0;
   if (dg) {
 #pragma unroll
-    for (int i = 0; i < L; ++i) s[0] += trapw(m0 + i, N) * d[i] * (v0 + (m0 + i) * dvv);
+    for (int i = 0; i < L; ++i) s[0] += chunk_w<TPT, L>(i, t) * d[i] * (vb + (double)i * dvv);
     row_sum<TPT, 1>(c, s, rs);
     vbar = dv * s[0];
     s[0] = 0.0;
   }
 #pragma unroll
   for (int i = 0; i < L; ++i) {
-    const double w = (v0 + (m0 + i) * dvv) - vbar;
-    s[0] += trapw(m0 + i, N) * d[i] * (w * w);
+    const double w = (vb + (double)i * dvv) - vbar;
+    s[0] += chunk_w<TPT, L>(i, t) * d[i] * (w * w);
   }
   row_sum<TPT, 1>(c, s, rs);
   temp = dv * s[0];
@@ -405,17 +442,17 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
       CollCoef cf;
       cf.N = N;
       cf.set(nu_dt, -temp / dv2, 0.5 / a.dv, a.v0, a.dvv, vbar, 1.0 + nu_dt * (2.0 * temp / dv2));
-      tridiag<TPT, L>(c, d, cf, t, red);
+      tridiag<TPT, L, CollCoef, Ctx, sizeof(T) == 8>(c, d, cf, t, red);
     }
     if (a.flags & RK_MOMENTS) {
       double s[kMoments];
 #pragma unroll
       for (int q = 0; q < kMoments; ++q) s[q] = 0.0;
+      const double vb = a.v0 + (double)m0 * a.dvv;
 #pragma unroll
       for (int i = 0; i < L; ++i) {
-        const int m = m0 + i;
-        const double wf = trapw(m, N) * d[i];
-        const double vm = a.v0 + m * a.dvv;
+        const double wf = chunk_w<TPT, L>(i, t) * d[i];
+        const double vm = vb + (double)i * a.dvv;
         double p = wf;
         s[0] += p;
         p *= vm;
@@ -628,6 +665,12 @@ struct DriftArgs {
   // replaces dk
   const double* dk_inst;
   int xcd_order;  // 1: consecutive column tiles go to workgroups on one XCD
+  // FP32, nx in {32, 64}: the x-transforms as dense real-DFT products on
+  // MFMA (vpfp_drift.hip k_drift_dft) with these matrices, [2][nx][nx]:
+  // forward rows k <= nx/2 cos(2 pi k x / nx), rows nx/2 + j -sin(2 pi j x / nx)
+  // (0 < j < nx/2); inverse [x][k] the matching real synthesis / nx.
+  // Null: the Stockham transform (drift_body).
+  const float* dft;
 };
 
 // Workgroup b of n runs on XCD b % 8 (round-robin dispatch).  With
@@ -670,10 +713,14 @@ VP_HD void drift_body(Ctx& c, unsigned char* lds, const DriftArgs<T>& a) {
 #pragma unroll
   for (int i = 0; i < 2 * Q; ++i) acc[i] = 0.0;
   const int ntiles = a.nv / C;
-  // the grid strides over the column tiles; workgroup b accumulates the
-  // density of every tile it advects into one partial row dens[b][x]
+  // workgroup b takes the run of `per` consecutive column tiles starting at
+  // b per (per = ntiles / workgroups rounded up): neighbouring narrow tiles
+  // share 128-byte lines, which one workgroup then fetches into one L2;
+  // it accumulates the density of its tiles into one partial row dens[b][x]
   const int b0 = a.xcd_order ? xcd_block(c.bid, c.nblk) : c.bid;
-  for (int tile = b0; tile < ntiles; tile += c.nblk) {
+  const int per = (ntiles + c.nblk - 1) / c.nblk;
+  const int t0 = b0 * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
+  for (int tile = t0; tile < t1; ++tile) {
     const int col = tile * C + cc;
     const T* fi = a.f_in + foff + col;
     cx<T> v[Q];
@@ -727,6 +774,151 @@ VP_HD void drift_body(Ctx& c, unsigned char* lds, const DriftArgs<T>& a) {
       field_body<M, Cfg::fq, Cfg::wg, true>(c, lds, fa);
       if (c.tid == 0) c.reset(fa.counter);
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// x-advection of FP32 columns as two dense real-DFT products on MFMA
+// (nx = NX in {32, 64}).  Each wave takes a tile of 16 columns:
+//   Y = F X      (F: [NX][NX] forward real DFT, X: [NX][16] the columns)
+//   phase        (Re_k, Im_k) of column c times exp(-i k phi_c)
+//   X' = G Y'    (G: [NX][NX] inverse real DFT)
+// through the context's 16 x 16 x 4 FP32 matrix product (c.mfma_16x16x4:
+// D = A B + C, A 16x4, B 4x16; lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16] and holds D[4 (l / 16) + r][l % 16], r < 4).  Rows of Y:
+// Re_k for k <= NX/2 (row NX/2 the Nyquist bin), Im_k at row NX/2 + k
+// (0 < k < NX/2), so a lane holds Re_k and Im_k of one column in tiles t and
+// t + NX/32 at the same position.  The phase is the Stockham drift's:
+// phi = (dk dt) v_c, exp(-i k phi) for 0 < k < NX/2 and cos((NX/2) phi) on
+// the Nyquist bin (the real part the reference keeps).  The B operand of the
+// second product is re-laid through a per-wave LDS tile.  W waves per
+// workgroup; the workgroups stride over groups of W tiles (the tile count a
+// multiple of W, so every wave makes the same number of passes).
+// ---------------------------------------------------------------------------
+struct f4 {
+  float v[4];
+};
+
+template <int NX, int W>
+struct DftCfg {
+  static constexpr int T = NX / 16;   // 16-row tiles of the products
+  static constexpr int KS = NX / 4;   // k-steps of 4
+  static constexpr int GP = 17;       // pitch (floats) of a wave's Y' tile
+  static constexpr int mat = 2 * NX * NX * 4;
+  static constexpr int gt = NX * GP * 4;
+  static constexpr int lds = mat + W * gt + W * NX * 8;
+};
+
+template <int NX, int W, class Ctx>
+VP_HD void drift_dft_body(Ctx& c, unsigned char* lds, const DriftArgs<float>& a) {
+  using Cfg = DftCfg<NX, W>;
+  constexpr int T = Cfg::T, KS = Cfg::KS, GP = Cfg::GP;
+  float* Fw = reinterpret_cast<float*>(lds);
+  float* Iv = Fw + NX * NX;
+  const int lane = c.tid & 63, w = c.tid >> 6;
+  float* Y = reinterpret_cast<float*>(lds + Cfg::mat) + w * NX * GP;
+  double* red = reinterpret_cast<double*>(lds + Cfg::mat + W * Cfg::gt);
+  for (int i = c.tid; i < 2 * NX * NX; i += c.nthr) Fw[i] = a.dft[i];
+  const int li = lane & 15, lk = lane >> 4;
+  const size_t nv = (size_t)a.nv;
+  const int inst = c.bidy;
+  const size_t foff = (size_t)inst * NX * nv;
+  const double dk = a.dk_inst ? a.dk_inst[inst] : a.dk;
+  double dacc[T][4];
+#pragma unroll
+  for (int t = 0; t < T; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dacc[t][r] = 0.0;
+  // workgroup b takes a run of consecutive passes (neighbouring 16-column
+  // tiles share 128-byte lines)
+  const int groups = (a.nv / 16) / W;
+  const int per = (groups + c.nblk - 1) / c.nblk;
+  const int g0 = c.bid * per, g1 = g0 + per < groups ? g0 + per : groups;
+  c.sync();  // matrices staged
+  for (int gi = g0; gi < g1; ++gi) {
+    const int col = (gi * W + w) * 16 + li;
+    const float* fi = a.f_in + foff + col;
+    f4 acc[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) acc[t] = f4{{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const float xb = fi[(size_t)(4 * s + lk) * nv];
+#pragma unroll
+      for (int t = 0; t < T; ++t) acc[t] = c.mfma_16x16x4(Fw[(16 * t + li) * NX + 4 * s + lk], xb, acc[t]);
+    }
+    // phase: bins k = 16 t + 4 lk + r (t < T/2) by a running product,
+    // exp(-i k phi) = exp(-i 4 lk phi) exp(-i r phi) exp(-i 16 t phi)
+    const double phi = (dk * a.dt) * a.v[col];
+    cx<float> p0, p1, p16;
+    cis_neg((double)(4 * lk) * phi, p0);
+    cis_neg(phi, p1);
+    cis_neg(16.0 * phi, p16);
+    cx<float> pt = p0;
+#pragma unroll
+    for (int t = 0; t < T / 2; ++t) {
+      cx<float> pk = pt;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float re = acc[t].v[r], im = acc[t + T / 2].v[r];
+        if (t == 0 && r == 0 && lk == 0) {
+          // DC unchanged; the partner row holds the real Nyquist bin
+          cx<float> pn;
+          cis_neg((double)(NX / 2) * phi, pn);
+          acc[T / 2].v[0] = pn.re * im;
+        } else {
+          acc[t].v[r] = re * pk.re - im * pk.im;
+          acc[t + T / 2].v[r] = im * pk.re + re * pk.im;
+        }
+        pk = mul(pk, p1);
+      }
+      pt = mul(pt, p16);
+    }
+    // Y' to the wave's LDS tile, rows 16 t + 4 lk + r, column li
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Y[(16 * t + 4 * lk + r) * GP + li] = acc[t].v[r];
+    c.sync();
+    f4 out[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) out[t] = f4{{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const float yb = Y[(4 * s + lk) * GP + li];
+#pragma unroll
+      for (int t = 0; t < T; ++t) out[t] = c.mfma_16x16x4(Iv[(16 * t + li) * NX + 4 * s + lk], yb, out[t]);
+    }
+    c.sync();  // Y read by every lane before the next pass writes it
+    float* fo = a.f_out + foff + col;
+    const double wc = (col == 0 || col == a.nv - 1) ? 0.5 : 1.0;
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        fo[(size_t)(16 * t + 4 * lk + r) * nv] = out[t].v[r];
+        dacc[t][r] += wc * (double)out[t].v[r];
+      }
+  }
+  if (!a.dens) return;
+  // partial density of this workgroup: sum over the 16 columns of a lane
+  // group, then over the waves
+#pragma unroll
+  for (int t = 0; t < T; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      double v = dacc[t][r];
+#pragma unroll
+      for (int m = 1; m < 16; m <<= 1) v += c.shfl_xor(v, m, 16);
+      if (li == 0) red[w * NX + 16 * t + 4 * lk + r] = v;
+    }
+  c.sync();
+  double* out = a.dens + ((size_t)inst * c.nblk + c.bid) * NX;
+  for (int x = c.tid; x < NX; x += c.nthr) {
+    double sum = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < W; ++ww) sum += red[ww * NX + x];
+    out[x] = sum;
   }
 }
 
@@ -842,7 +1034,7 @@ VP_HD void tri_body(Ctx& c, unsigned char* lds, const TriArgs<T>& a) {
   cf.bb = a.b + (size_t)rowc * N;
   cf.cc = a.cc + (size_t)rowc * (N - 1);
   cf.N = N;
-  tridiag<TPT, L>(c, d, cf, t, red);
+  tridiag<TPT, L, ArrCoef, Ctx, sizeof(T) == 8>(c, d, cf, t, red);
   if (valid) {
     T* xr = a.x + (size_t)row * N;
 #pragma unroll

@@ -31,6 +31,13 @@ struct GpuCtx {
   }
   __device__ __forceinline__ double shfl_xor(double v, int m, int w) { return __shfl_xor(v, m, w); }
   __device__ __forceinline__ float shfl_xor(float v, int m, int w) { return __shfl_xor(v, m, w); }
+  // v_mfma_f32_16x16x4_f32: D = A B + C (layout at vp::drift_dft_body)
+  __device__ __forceinline__ vp::f4 mfma_16x16x4(float a, float b, vp::f4 acc) {
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    f4v cv = {acc.v[0], acc.v[1], acc.v[2], acc.v[3]};
+    cv = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, cv, 0, 0, 0);
+    return vp::f4{{cv[0], cv[1], cv[2], cv[3]}};
+  }
 };
 
 #define VPFP_CTX \

@@ -112,6 +112,7 @@ class _Class:
         tu = _lib.Tuning()
         tu.row_q, tu.drift_tile, tu.fuse_field = int(t["row_q"]), int(t["drift_tile"]), 0
         tu.drift_order, tu.drift_tiles = int(t["drift_order"]), int(t["drift_tiles"])
+        tu.drift_dft = int(t.get("drift_dft", 0))
         _lib.check(_lib.library().vpfp_batch_set_tuning(self.handle, ctypes.byref(tu)))
 
     def set_profiling(self, on):
@@ -184,10 +185,11 @@ class Scan:
     def shapes(self):
         """Grouped scans: the group's launch shapes per class {(nx, nv): tuning}."""
         nb = len(self.classes)
-        q, C, per = (ctypes.c_int * nb)(), (ctypes.c_int * nb)(), (ctypes.c_int * nb)()
-        _lib.check(_lib.library().vpfp_group_get_shapes(self.group, q, C, per))
+        q, C, per, dft = ((ctypes.c_int * nb)() for _ in range(4))
+        _lib.check(_lib.library().vpfp_group_get_shapes(self.group, q, C, per, dft))
         return {(c.nx, c.nv): {"row_q": q[i], "drift_tile": C[i], "fuse_field": 0, "drift_order": 0,
-                               "drift_tiles": per[i], "us_per_step": 0.0} for i, c in enumerate(self.classes)}
+                               "drift_tiles": per[i], "us_per_step": 0.0, "drift_dft": dft[i]}
+                for i, c in enumerate(self.classes)}
 
     def autotune(self, n_steps=5):
         """Pick the launch shapes by timing them (scratch state; the instances
@@ -215,7 +217,8 @@ class Scan:
             q = (ctypes.c_int * nb)(*[int(x["row_q"]) for x in t])
             C = (ctypes.c_int * nb)(*[int(x["drift_tile"]) for x in t])
             per = (ctypes.c_int * nb)(*[max(1, int(x["drift_tiles"])) for x in t])
-            _lib.check(_lib.library().vpfp_group_set_shapes(self.group, q, C, per))
+            dft = (ctypes.c_int * nb)(*[int(x.get("drift_dft", 0)) for x in t])
+            _lib.check(_lib.library().vpfp_group_set_shapes(self.group, q, C, per, dft))
             return
         for c in self.classes:
             c.set_tuning(tuning[(c.nx, c.nv)])
