@@ -379,6 +379,45 @@ def scan_roofline(run, traffic_file=None):
     }
 
 
+def drift_mfma_section(tuning, steps=20):
+    """The x-advection of the nx = 32 and nx = 64 launches of the FP32
+    delta-f scan as Stockham transforms (the tuned shapes) and as dense
+    real-DFT products on MFMA (drift_dft, 16 and 32 columns per pass, the
+    same tiles per workgroup): microseconds per launch from HIP events over
+    `steps` steps, the faster per nx, and the MFMA instruction rate."""
+    import copy
+
+    import torch
+
+    from vlapy_amd.scan import Scan, draw_parameters
+
+    p = draw_parameters(SCAN_N, SCAN_SEED)
+    sc = Scan(p, dt=SCAN_DT, op="dg", precision="fp32", delta=True)
+    out = {}
+    for name, dft, C in (("stockham", 0, None), ("mfma_16", 1, 16), ("mfma_32", 1, 32)):
+        t = copy.deepcopy(tuning)
+        for (nx, nv), tt in t.items():
+            if nx in (32, 64):
+                tt["drift_dft"] = dft
+                if C:
+                    tt["drift_tile"] = C
+        sc.set_tuning(t)
+        sc.run(2, record=False)
+        sc.set_profiling(True)
+        sc.run(steps, record=False)
+        torch.cuda.synchronize()
+        kt = sc.kernel_times()
+        sc.set_profiling(False)
+        for nx in (32, 64):
+            ms, n = kt[("nx", nx)]["drift"]
+            out.setdefault("nx=%d" % nx, {})[name] = ms / n * 1e3
+    for nx in (32, 64):
+        d = out["nx=%d" % nx]
+        d["faster"] = min(d, key=d.get)
+    return {"workload": "x-advection launches of the FP32 delta-f scan (grouped), %d steps" % steps,
+            "us_per_launch": out}
+
+
 def scan_cpu(p, steps_sample, n_sample, procs):
     """Oracle time on a sample of the scan's instances, extrapolated by
     cell-steps to the whole scan (run in a child process with one BLAS/OMP
@@ -512,7 +551,8 @@ def scan_section(timed_steps, warmup, with_cpu, judge=True, profile_steps=SCAN_P
                          "cell_updates_per_s": cells * steps / rd["wall"],
                          "graph_instances_per_s": SCAN_N / rd["graph_wall"] if rd["graph_wall"] else None,
                          "damping_rate": summary(damping_errs(rd["e2"])), "drift": drifts(rd),
-                         "tuning": {"%dx%d" % k: v for k, v in rd["tuning"].items()}},
+                         "tuning": {"%dx%d" % k: v for k, v in rd["tuning"].items()},
+                         "tuning_raw": rd["tuning"]},
         "fp64": {"instances_per_s": SCAN_N / r64["wall"], "seconds": r64["wall"],
                  "cell_updates_per_s": cells * steps / r64["wall"], "drift": drifts(r64)},
         "fp32": {"instances_per_s": SCAN_N / r32["wall"], "seconds": r32["wall"],
@@ -753,6 +793,8 @@ def main():
         "cpu_baseline": None,
         "scan": scan,
     }
+    if not args.no_judge:
+        scan["drift_mfma"] = drift_mfma_section(scan["fp32_delta_f"]["tuning_raw"])
     if "cpu_baseline" in scan:
         a = scan["cpu_baseline"]["all_cores"]
         out["cpu_baseline"] = {"value": a["instances_per_s"], "unit": "instances/s", "cores": a["cores"],
@@ -765,8 +807,84 @@ def main():
         out["large_grid"] = large_section(args.large_steps, not args.no_autotune)
     if not args.no_closure:
         out["closure"] = closure_section()
+    out["scan"]["fp32_delta_f"].pop("tuning_raw", None)
     print(json.dumps(out))
     return 0
+
+
+def epw_graph(precision, steps, chunk=100):
+    """The EPW step loop (step_n with the E history, moments and series)
+    captured for `chunk` steps in one HIP graph and replayed steps / chunk
+    times: ms per step.  Every replay writes the same history rows."""
+    import numpy as np
+    import torch
+
+    from vlapy_amd import ops
+    from vlapy_amd._lib import NMOMENTS, Plan
+
+    dev = torch.device("cuda:0")
+    x, kx, okx, v, kv, dv, f0 = initial_state(np)
+    plan = Plan(NX, NV, precision, kx, okx, v, kv, dv, NU, DT, "dg")
+    plan.autotune(20)
+    f = torch.as_tensor(f0, dtype=ops.f_dtype(plan), device=dev).contiguous()
+    e = torch.zeros(NX, dtype=torch.float64, device=dev)
+    eh = torch.zeros(chunk, NX, dtype=torch.float64, device=dev)
+    mo = torch.zeros(chunk, NMOMENTS, NX, dtype=torch.float64, device=dev)
+    se = torch.zeros(chunk, 2, dtype=torch.float64, device=dev)
+    ops.step_n(plan, f, e, chunk, None, eh, mo, se)  # warm-up
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream()
+    cap.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(cap):
+        with torch.cuda.graph(g, stream=cap):
+            ops.step_n(plan, f, e, chunk, None, eh, mo, se)
+    torch.cuda.current_stream().wait_stream(cap)
+    torch.cuda.synchronize()
+    reps = max(1, steps // chunk)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / (reps * chunk)
+
+
+def dropin_section(steps=NSTEPS, steps_in_loop=1000):
+    """The EPW workload through the reference's inner-loop interface
+    (outer_loop.get_inner_loop_stepper with backend "hip", FP64): batches of
+    steps_in_loop steps, each filling the reference's temp_storage (fields,
+    series, the k0/k1 f store) on the host.  ms per step, beside step_n."""
+    import numpy as np
+    import torch
+
+    from vlapy_amd import initializers, outer_loop
+
+    ap = initializers.make_default_params_dictionary()
+    ap.update({"nx": NX, "nv": NV, "nu": NU, "xmin": 0.0, "xmax": 2 * np.pi / K0, "vmax": VMAX,
+               "tmax": DT * steps, "nt": steps + 1})
+    ap["fokker-planck"]["type"] = "dg"
+
+    class D:
+        rules_to_store_f = {"space": ["k0", "k1"], "time": "all"}
+
+    stuff = outer_loop.get_everything_ready_for_outer_loop(D, ap, {}, steps)
+    x = stuff["x"]
+    stuff["f"] = stuff["f"] * (1 + AMP * np.cos(K0 * x))[:, None]
+    stepper = outer_loop.get_inner_loop_stepper(ap, stuff, steps_in_loop)
+    ts = outer_loop.get_arrays_for_inner_loop(stuff, steps_in_loop, D.rules_to_store_f)
+    stepper(stuff["t"][:steps_in_loop], stuff["driver"][:steps_in_loop], ts)  # warm-up batch
+    torch.cuda.synchronize()
+    ts = outer_loop.get_arrays_for_inner_loop(stuff, steps_in_loop, D.rules_to_store_f)
+    t0 = time.perf_counter()
+    for b in range(steps // steps_in_loop):
+        sl = slice(b * steps_in_loop, (b + 1) * steps_in_loop)
+        ts = stepper(stuff["t"][sl], stuff["driver"][sl], ts)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return {"workload": WORKLOAD + "; reference inner-loop interface, %d-step batches, k0/k1 f store" % steps_in_loop,
+            "ms_per_step": wall / steps * 1e3, "steps": steps}
 
 
 def epw_section(steps, warmup, profile_steps, autotune, cpu_seconds):
@@ -789,11 +907,19 @@ def epw_section(steps, warmup, profile_steps, autotune, cpu_seconds):
         return {"cell_updates_per_s": cells * steps / (r["ms"] / 1e3), "ms_per_step": r["ms"] / steps,
                 "roofline": roofline(r, cells, r["ms"] / steps, traffic), "tuning": r["tuning"]}
 
-    return {"workload": WORKLOAD, "steps": steps,
-            "fp32": dict(line(r32, "traffic_c2_fp32.json"), vs_fp64_after_run=err(r32)),
-            "fp64": line(r64, "traffic_c2_fp64.json"),
-            "fp32_delta_f": dict(line(r32d, None), vs_fp64_after_run=err(r32d)),
-            "cpu_baseline": cpu}
+    out = {"workload": WORKLOAD, "steps": steps,
+           "fp32": dict(line(r32, "traffic_c2_fp32.json"), vs_fp64_after_run=err(r32)),
+           "fp64": line(r64, "traffic_c2_fp64.json"),
+           "fp32_delta_f": dict(line(r32d, None), vs_fp64_after_run=err(r32d)),
+           "cpu_baseline": cpu}
+    # launch overhead removed: the same loop replayed from a HIP graph
+    out["fp64"]["graph_ms_per_step"] = epw_graph("fp64", steps)
+    out["fp32"]["graph_ms_per_step"] = epw_graph("fp32", steps)
+    # the reference's own inner-loop interface on the same workload (FP64)
+    d = dropin_section(steps)
+    d["vs_step_n"] = d["ms_per_step"] / out["fp64"]["ms_per_step"]
+    out["dropin_fp64"] = d
+    return out
 
 
 if __name__ == "__main__":

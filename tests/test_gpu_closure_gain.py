@@ -9,9 +9,14 @@ grid with the closure after every step.  Printed: median relative error of
 log10 <E^2>(t) and of the damping rate against the fine grid, and the
 fraction of runs within 1 % of the fine damping rate, for both coarse runs.
 Asserted: the closure run stays finite, adds no density beyond the plain
-coarse run's own FP32 drift, and its errors stay within 1.5x of the plain
-coarse run's (the trained closure of this configuration does not improve on
-the plain coarse grid; the numbers printed are the measurement).
+coarse run's own FP32 drift, and the measured relation holds: the trained
+closure of this configuration gives no consistent gain over the plain coarse
+grid (history error between 0.9x and 1.25x of the plain run's, measured
+0.048 vs 0.041 on these 32 runs and 0.076 vs 0.074 on 512; median damping
+error within 1.6x of it, measured 0.135 % vs 0.140 % here and 0.19 % vs
+0.14 % on 512), while delta-f storage of the same coarse grid, with no
+network, does gain (history error below half the plain run's, measured
+0.013).
 """
 
 import os
@@ -48,5 +53,11 @@ def test_closure_against_plain_coarse_grid():
         print(name, "history err median %.4f damping err median %.5f within 1%% %.3f" % res[name], flush=True)
     print("density drift max: plain %.2e closure %.2e" % (drift_plain, drift_clo), flush=True)
     assert drift_clo <= max(2 * drift_plain, 1e-6)
-    assert res["closure"][0] <= 1.5 * res["plain"][0]
-    assert res["closure"][1] <= 1.5 * res["plain"][1] + 1e-3
+    # no gain from the network, and no large loss
+    assert 0.9 * res["plain"][0] <= res["closure"][0] <= 1.25 * res["plain"][0]
+    assert res["closure"][1] <= 1.6 * res["plain"][1] + 5e-4
+    # what does lower the error at this grid: f - M stored (no network)
+    dlt = CL.rollout(inst, CL.NV, "fp32", steps, None, True)
+    h_d = np.median(CL.history_errors(ref, dlt))
+    print("delta-f coarse history err median %.4f" % h_d, flush=True)
+    assert h_d < 0.5 * res["plain"][0]
