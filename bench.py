@@ -82,6 +82,44 @@ def cpu_child(target_s):
     print(json.dumps({"steps": steps, "seconds": el}))
 
 
+def oracle_child(arg):
+    """Time `steps` steps of the NumPy FP64 oracle on one configuration (one
+    thread): "c1:<steps>" the collisionless Landau run, "c4:<steps>" the
+    1024 x 8192 run with the Dougherty collision."""
+    import numpy as np
+
+    sys.path.insert(0, HERE)
+    from oracle import vpfp_oracle as O
+
+    cfg, steps = arg.split(":")
+    steps = int(steps)
+    if cfg == "c1":
+        x, kx, okx, v, kv, dv, f = c1_state(np)
+        nu, dt = 0.0, C1_DT
+    else:
+        x, kx, okx, v, kv, dv, f = large_state(np)
+        nu, dt = LG_NU, LG_DT
+    g = O.Grid(x, kx, okx, v, kv, dv)
+    e = np.zeros(x.size)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        e, f = O.leapfrog_step(g, e, f, i * dt, dt)
+        if nu > 0.0:
+            f = O.collide(f, v, nu, dt, dv, "dg")
+    print(json.dumps({"steps": steps, "seconds": time.perf_counter() - t0}))
+
+
+def oracle_time(cfg, steps):
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
+        env[k] = "1"
+    res = subprocess.run([sys.executable, os.path.abspath(__file__), "--oracle-child", "%s:%d" % (cfg, steps)],
+                         capture_output=True, text=True, env=env, timeout=600)
+    if res.returncode != 0:
+        raise RuntimeError("oracle timing failed: " + res.stderr[-2000:])
+    return json.loads(res.stdout.strip().splitlines()[-1])
+
+
 def cpu_baseline(target_s):
     env = dict(os.environ)
     for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS", "NUMEXPR_NUM_THREADS"):
@@ -589,6 +627,87 @@ def scan_section(timed_steps, warmup, with_cpu, judge=True, profile_steps=SCAN_P
 LG_NX, LG_NV, LG_K, LG_VMAX, LG_AMP, LG_NU, LG_DT = 1024, 8192, 0.35, 8.0, 0.1, 1e-4, 0.02
 
 
+C1_NX, C1_NV, C1_K, C1_VMAX, C1_AMP, C1_DT, C1_STEPS = 64, 512, 0.3, 6.4, 1e-4, 0.1, 1000
+
+
+def c1_state(np):
+    """Linear Landau damping: nx=64 on [0, 2 pi/k), k=0.3; nv=512 on [-6.4, 6.4];
+    f0 = (1 + 1e-4 cos kx) exp(-v^2/2)/sqrt(2 pi); e0 = 0; no collisions."""
+    xmax = 2 * np.pi / C1_K
+    dx = xmax / C1_NX
+    x = np.linspace(dx / 2.0, xmax - dx / 2.0, C1_NX)
+    kx = np.fft.fftfreq(C1_NX, d=dx) * 2.0 * np.pi
+    okx = np.zeros_like(kx)
+    okx[1:] = 1.0 / kx[1:]
+    dv = 2 * C1_VMAX / C1_NV
+    v = np.linspace(-C1_VMAX + dv / 2.0, C1_VMAX - dv / 2.0, C1_NV)
+    kv = np.fft.fftfreq(C1_NV, d=dv) * 2.0 * np.pi
+    f0 = (1.0 + C1_AMP * np.cos(C1_K * x))[:, None] * (np.exp(-v ** 2 / 2.0) / np.sqrt(2 * np.pi))[None, :]
+    return x, kx, okx, v, kv, dv, f0
+
+
+def c1_section(with_cpu):
+    """The collisionless Landau run, FP64, 1000 leapfrog steps with the E
+    history: ms per step (HIP events; eager and from a 100-step HIP graph),
+    the damping rate and frequency of <E^2> against the Landau root, and the
+    oracle's time for the same 1000 steps on one core."""
+    import numpy as np
+    import torch
+
+    from vlapy_amd import ops
+    from vlapy_amd._lib import Plan
+    from vlapy_amd.diagnostics.low_level_helpers import get_standing_wave_rate_and_frequency
+
+    dev = torch.device("cuda:0")
+    x, kx, okx, v, kv, dv, f0 = c1_state(np)
+    plan = Plan(C1_NX, C1_NV, "fp64", kx, okx, v, kv, dv, 0.0, C1_DT, "lb")
+    tuning = plan.autotune(50)
+    f = torch.as_tensor(f0, device=dev).contiguous()
+    e = torch.zeros(C1_NX, dtype=torch.float64, device=dev)
+    eh = torch.zeros(C1_STEPS, C1_NX, dtype=torch.float64, device=dev)
+    ops.step_n(plan, f, e, 20, None, eh[:20])  # warm-up
+    f.copy_(torch.as_tensor(f0, device=dev))
+    e.zero_()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    ops.step_n(plan, f, e, C1_STEPS, None, eh)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / C1_STEPS
+    e2 = (eh ** 2).mean(dim=1).cpu().numpy()
+    t = C1_DT * (np.arange(C1_STEPS) + 1)
+    gamma, omega = get_standing_wave_rate_and_frequency(e2, t, t_min=8.0)
+    # the same loop replayed from a HIP graph of 100 steps
+    chunk = 100
+    g = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream()
+    cap.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(cap):
+        with torch.cuda.graph(g, stream=cap):
+            ops.step_n(plan, f, e, chunk, None, eh[:chunk])
+    torch.cuda.current_stream().wait_stream(cap)
+    torch.cuda.synchronize()
+    a.record()
+    for _ in range(C1_STEPS // chunk):
+        g.replay()
+    b.record()
+    torch.cuda.synchronize()
+    graph_ms = a.elapsed_time(b) / C1_STEPS
+    out = {"workload": "collisionless Landau damping: nx=64, nv=512, v in [-6.4, 6.4], k=0.3, density "
+                       "1+1e-4cos(kx), e0=0, dt=0.1, 1000 leapfrog steps with the E history, FP64",
+           "ms_per_step": ms, "cell_updates_per_s": C1_NX * C1_NV / (ms / 1e3), "graph_ms_per_step": graph_ms,
+           "tuning": tuning, "gamma": gamma, "omega": omega,
+           "landau_root": {"gamma": -0.012623, "omega": 1.1598}}
+    if with_cpu:
+        r = oracle_time("c1", C1_STEPS)
+        out["cpu_baseline"] = {"value": C1_NX * C1_NV * r["steps"] / r["seconds"], "unit": "cell-updates/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "the same 1000 steps on the NumPy FP64 oracle, %.1f s, one thread"
+                                         % r["seconds"]}
+    return out
+
+
 def large_state(np):
     """nx=1024 on [0, 2 pi/k), k=0.35; nv=8192 on [-8, 8]; f0 = (1 + 0.1 sum_{m=1..3}
     cos(m k x + phi_m)) exp(-v^2/2)/sqrt(2 pi), phi_m ~ U[0, 2 pi) from default_rng(4)."""
@@ -607,9 +726,10 @@ def large_state(np):
     return x, kx, okx, v, kv, dv, f0
 
 
-def large_section(steps, autotune=True):
+def large_section(steps, autotune=True, with_cpu=False):
     """Single high-resolution run (8.4M cells), Dougherty nu=1e-4, dt=0.02:
-    `steps` leapfrog + collision steps with the E history, FP32 and FP64."""
+    `steps` leapfrog + collision steps with the E history, FP32 and FP64;
+    with_cpu: the oracle's time for 2 steps on one core."""
     import numpy as np
     import torch
 
@@ -651,6 +771,12 @@ def large_section(steps, autotune=True):
         out[prec] = {"cell_updates_per_s": cells / (ms / 1e3), "ms_per_step": ms, "tuning": tuning,
                      "roofline": roofline(run, cells, ms, "traffic_large_%s.json" % prec)}
         del plan, f
+    if with_cpu:
+        r = oracle_time("c4", 2)
+        out["cpu_baseline"] = {"value": cells * r["steps"] / r["seconds"], "unit": "cell-updates/s", "cores": 1,
+                               "kind": "port",
+                               "sample": "the first 2 steps on the NumPy FP64 oracle (leapfrog + Dougherty "
+                                         "collision), %.1f s, one thread" % r["seconds"]}
     return out
 
 
@@ -733,10 +859,14 @@ def main():
     ap.add_argument("--scan-profile-steps", type=int, default=SCAN_PROFILE_STEPS)
     ap.add_argument("--cpu-child", type=float, default=None, help=argparse.SUPPRESS)
     ap.add_argument("--scan-cpu-child", type=str, default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--oracle-child", type=str, default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     if args.cpu_child is not None:
         cpu_child(args.cpu_child)
+        return 0
+    if args.oracle_child is not None:
+        oracle_child(args.oracle_child)
         return 0
     if args.scan_cpu_child is not None:
         scan_cpu_child(args.scan_cpu_child)
@@ -804,7 +934,9 @@ def main():
         out["epw"] = epw_section(args.epw_steps, args.epw_warmup, min(args.profile_steps, args.epw_steps),
                                  not args.no_autotune, None if args.no_cpu else args.cpu_seconds)
     if not args.no_large:
-        out["large_grid"] = large_section(args.large_steps, not args.no_autotune)
+        out["large_grid"] = large_section(args.large_steps, not args.no_autotune, not args.no_cpu)
+    if not args.no_epw:
+        out["landau_c1"] = c1_section(not args.no_cpu)
     if not args.no_closure:
         out["closure"] = closure_section()
     out["scan"]["fp32_delta_f"].pop("tuning_raw", None)
@@ -883,8 +1015,26 @@ def dropin_section(steps=NSTEPS, steps_in_loop=1000):
         ts = stepper(stuff["t"][sl], stuff["driver"][sl], ts)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # the device work of the same batches alone: step_n_store with the E
+    # history, moments and the k0/k1 store on the stepper's plan
+    from vlapy_amd import ops
+    from vlapy_amd._lib import NMOMENTS
+    dev = torch.device("cuda:0")
+    f = torch.as_tensor(stuff["f"], device=dev)
+    e = torch.zeros(NX, dtype=torch.float64, device=dev)
+    eh = torch.empty(steps_in_loop, NX, dtype=torch.float64, device=dev)
+    mo = torch.empty(steps_in_loop, NMOMENTS, NX, dtype=torch.float64, device=dev)
+    st = torch.empty(steps_in_loop, 2, NV, dtype=torch.complex128, device=dev)
+    ops.step_n_store(stepper.plan, f, e, steps_in_loop, st, 2, None, eh, mo)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for b in range(steps // steps_in_loop):
+        ops.step_n_store(stepper.plan, f, e, steps_in_loop, st, 2, None, eh, mo)
+    torch.cuda.synchronize()
+    dev_wall = time.perf_counter() - t1
     return {"workload": WORKLOAD + "; reference inner-loop interface, %d-step batches, k0/k1 f store" % steps_in_loop,
-            "ms_per_step": wall / steps * 1e3, "steps": steps}
+            "ms_per_step": wall / steps * 1e3, "steps": steps,
+            "step_n_store_ms_per_step": dev_wall / steps * 1e3, "vs_step_n_store": wall / dev_wall}
 
 
 def epw_section(steps, warmup, profile_steps, autotune, cpu_seconds):

@@ -152,9 +152,12 @@ int vpfp_step_n(vpfp_plan_t plan, void* f, double* e, int n_steps, const double*
  *                 (written by the row kernel between the collision and the
  *                 next half kick, no extra launch);
  *   n_modes > 0:  the first n_modes Fourier modes along x of f, f_store
- *                 [n_steps][n_modes][nv] complex128 as vpfp_x_modes (one
- *                 extra launch per step on a copy the row kernel writes).
- * Under delta-f storage the store holds f itself. */
+ *                 [n_steps][n_modes][nv] complex128, equal to vpfp_x_modes:
+ *                 the row kernel writes f to plan scratch and extra
+ *                 workgroups of the next step's field launch sum it over x
+ *                 (one more launch after the last step).
+ * Three launches per step either way.  Under delta-f storage the store holds
+ * f itself. */
 int vpfp_step_n_store(vpfp_plan_t plan, void* f, double* e, int n_steps, const double* driver, double* e_hist,
                       double* moments, double* series, void* f_store, int n_modes, void* stream);
 

@@ -198,18 +198,21 @@ def test_step_n_matches_oracle_loop(dev, nx, nv, nu, op, n):
     assert rel(e.cpu().numpy(), ef) < 1e-11
 
 
-@pytest.mark.parametrize("nu,delta,n_modes", [(1e-2, False, 0), (1e-2, False, 3), (0.0, False, 0), (1e-2, True, 0),
-                                               (1e-2, True, 2)])
-def test_step_n_store_matches_stepwise_copies(dev, nu, delta, n_modes):
+@pytest.mark.parametrize("nu,delta,n_modes,nx,nv", [(1e-2, False, 0, 32, 256), (1e-2, False, 3, 32, 256),
+                                                     (0.0, False, 0, 32, 256), (0.0, False, 2, 32, 256),
+                                                     (1e-2, True, 0, 32, 256), (1e-2, True, 2, 32, 256),
+                                                     (1e-3, False, 2, 64, 1024)])
+def test_step_n_store_matches_stepwise_copies(dev, nu, delta, n_modes, nx, nv):
     """vpfp_step_n_store over n steps equals n single steps of vpfp_step_n
-    with f (or its first x-modes) taken after each: the copy the row kernel
-    writes between the collision and the next half kick is f after the step
-    (f itself under delta-f storage)."""
+    with f (or its first x-modes from vpfp_x_modes) taken after each: the
+    row kernel writes f between the collision and the next half kick (f
+    itself under delta-f storage) and the next step's field launch, or one
+    launch after the last step, sums its x-modes in vpfp_x_modes' order."""
     import torch
     from vlapy_amd import ops
 
-    nx, nv, k, dt, n = 32, 256, 0.35, 0.1, 6
-    print("step_n_store nu", nu, "delta", delta, "modes", n_modes, flush=True)
+    k, dt, n = 0.35, 0.1, 6
+    print("step_n_store nu", nu, "delta", delta, "modes", n_modes, "grid", nx, nv, flush=True)
     g = O.Grid.make(nx, nv, 2 * np.pi / k, 6.0)
     f0 = O.perturbed_maxwellian(g.x, g.v, k, 0.2)
     m = np.exp(-g.v ** 2 / 2.0) / np.sqrt(2 * np.pi)
@@ -235,8 +238,9 @@ def test_step_n_store_matches_stepwise_copies(dev, nu, delta, n_modes):
         st = torch.zeros(n, nx, nv, dtype=torch.float64, device=dev)
     ops.step_n_store(p, f2, e2, n, st, n_modes)
     got = st.cpu().numpy()
+    tol = 1e-12 if delta else 1e-15
     for i in range(n):
-        assert rel(got[i], want[i]) < (1e-12 if delta else 1e-15), i
+        assert rel(got[i], want[i]) < tol, i
 
 
 def test_unsupported_sizes_raise(dev):

@@ -67,6 +67,19 @@ template <int M, int Q>
 __global__ __launch_bounds__(kFT) void k_field(vp::FieldArgs a) {
   extern __shared__ __align__(16) unsigned char lds[];
   VPFP_CTX;
+  if (a.xm_out) {
+    // the previous step's f store beside the solve: workgroups from
+    // xm_blk0 on take one (mode, v) pair per thread
+    if (c.bid >= a.xm_blk0) {
+      GpuCtx x{c.tid, c.nthr, c.bid - a.xm_blk0, c.nblk - a.xm_blk0, 0};
+      if (a.xm_f32)
+        vp::xmodes_body<float>(x, static_cast<const float*>(a.xm_f), a.nx, a.xm_nv, a.xm_n, a.xm_tw, a.xm_out);
+      else
+        vp::xmodes_body<double>(x, static_cast<const double*>(a.xm_f), a.nx, a.xm_nv, a.xm_n, a.xm_tw, a.xm_out);
+      return;
+    }
+    c.nblk = a.xm_blk0;
+  }
   vp::field_body<M, Q, kFT>(c, lds, vp::field_instance(a, c.bidy));
 }
 
@@ -174,8 +187,15 @@ int launch_field(const vp::FieldArgs& a, hipStream_t s, int n_inst) {
   using Cfg = vp::FieldCfg<M, Q, kFT>;
   auto kern = k_field<M, Q>;
   if (int rc = set_lds(kern, Cfg::lds)) return rc;
-  const dim3 grid = n_inst > 0 ? dim3(1, n_inst) : dim3(field_grid(a.nx));
-  hipLaunchKernelGGL(kern, grid, dim3(kFT), Cfg::lds, s, a);
+  vp::FieldArgs b = a;
+  int xm_blocks = 0;
+  if (b.xm_out) {
+    if (n_inst > 0) return fail(VPFP_EBADARG, "x-modes ride only on a single-grid field launch");
+    b.xm_blk0 = field_grid(a.nx);
+    xm_blocks = (b.xm_n * b.xm_nv + kFT - 1) / kFT;
+  }
+  const dim3 grid = n_inst > 0 ? dim3(1, n_inst) : dim3(field_grid(a.nx) + xm_blocks);
+  hipLaunchKernelGGL(kern, grid, dim3(kFT), Cfg::lds, s, b);
   return launched();
 }
 
@@ -353,12 +373,12 @@ struct vpfp_plan_s {
   void* tw_x = nullptr;          // cx<T> [lo | hi] tables for nx
   cx<double>* tw_xs = nullptr;   // cx<double> [lo | hi] tables for nx (field)
   cx<double>* tw_xd = nullptr;   // cx<double>[nx] full table (direct-DFT field)
+  void* fcopy = nullptr;         // [nx][nv] storage type: f after a step (vpfp_step_n_store's x-modes)
   double* dens = nullptr;        // [max(ntiles,1)][nx]
   double* ering = nullptr;       // [2][nx]
   double* rho_g = nullptr;       // [nx] field reduction scratch
   unsigned int* counter = nullptr;
   double* series = nullptr;      // [2]
-  void* fcopy = nullptr;         // [nx][nv] storage type: f after a step (vpfp_step_n_store's x-modes)
   Prof prof;  // per-kernel event timing (vpfp_plan_set_profiling)
   StreamLog streams;
   std::vector<void*> allocs;
@@ -567,6 +587,7 @@ vp::DriftArgs<T> drift_args(vpfp_plan_t p, const void* fin, void* fout, double d
 vp::FieldArgs field_args(vpfp_plan_t p, const double* part, int ntiles, double scale, const double* driver,
                          double* e_out, double* e_out2, double* series, bool pow2) {
   vp::FieldArgs a;
+  std::memset(&a, 0, sizeof(a));
   a.part = part;
   a.ntiles = ntiles;
   a.rho_g = p->rho_g;
@@ -654,9 +675,12 @@ int do_tri(vpfp_plan_t p, const double* A, const double* B, const double* Cc, co
   return tri_dispatch<T>(a, s);
 }
 
-// f_store (optional): per-step f store; n_modes == 0: [n][nx][nv] copies of
-// f (type T), n_modes > 0: [n][n_modes][nv] complex128 x-modes of f (from
-// the row kernel's copy in p->fcopy)
+// f_store (optional): per-step f store, written by the row kernel between
+// the collision and the next half kick; n_modes == 0: [n][nx][nv] copies of
+// f (type T), n_modes > 0: [n][n_modes][nv] complex128 x-modes of f, formed
+// from the row kernel's copy in p->fcopy by extra workgroups of the next
+// step's field launch (one k_xmodes launch after the last step, and per
+// step when the field is not a launch of its own)
 template <class T>
 int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, double* e_hist, double* moments,
               double* series, hipStream_t s, void* f_store = nullptr, int n_modes = 0) {
@@ -677,6 +701,15 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
     const double* drv = driver ? driver + (size_t)i * p->nx : nullptr;
     double* ser = series ? series + 2 * (size_t)i : nullptr;
     auto fa = field_args(p, p->dens, p->ntiles, p->dv, drv, ei, nullptr, ser, true);
+    auto* modes = reinterpret_cast<cx<double>*>(f_store);
+    if (f_store && n_modes > 0 && mode == 0 && i > 0) {
+      fa.xm_f = p->fcopy;
+      fa.xm_f32 = sizeof(T) == 4;
+      fa.xm_n = n_modes;
+      fa.xm_nv = p->nv;
+      fa.xm_tw = p->tw_xd;
+      fa.xm_out = modes + (size_t)(i - 1) * n_modes * p->nv;
+    }
     if (mode == 1) {
       auto da = drift_args<T>(p, f, f, p->dt, p->dens);
       da.fuse_field = 1;
@@ -702,12 +735,12 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
       rb.f_copy = n_modes > 0 ? reinterpret_cast<T*>(p->fcopy)
                               : reinterpret_cast<T*>(f_store) + (size_t)i * p->nx * p->nv;
     PROF(KK_ROW, row_dispatch<T>(rb, s, p->row_q));
-    if (f_store && n_modes > 0) {
-      auto* o = reinterpret_cast<cx<double>*>(f_store) + (size_t)i * n_modes * p->nv;
+    if (f_store && n_modes > 0 && (mode != 0 || i + 1 == n)) {
       const int total = n_modes * p->nv;
       if (int rc = prof_mark(p->prof, KK_OTHER, s)) return rc;
       hipLaunchKernelGGL(k_xmodes<T>, dim3((total + 255) / 256), dim3(256), 0, s,
-                         reinterpret_cast<const T*>(p->fcopy), p->nx, p->nv, n_modes, p->tw_xd, o);
+                         reinterpret_cast<const T*>(p->fcopy), p->nx, p->nv, n_modes, p->tw_xd,
+                         modes + (size_t)i * n_modes * p->nv);
       if (int rc = launched()) return rc;
       if (int rc = prof_mark(p->prof, KK_OTHER, s)) return rc;
     }

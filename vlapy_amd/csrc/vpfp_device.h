@@ -327,12 +327,22 @@ struct PhaseMul {
 
 // Poisson multiplier i * one_over_kx (zero real part at DC and Nyquist);
 // okx is an LDS copy of the one_over_kx table.
-// exp(-i theta) - 1 without cancellation: (-2 sin^2(theta/2), -sin theta)
+// exp(-i theta) - 1 without cancellation: (-2 sin^2(theta/2), -2 sin(theta/2)
+// cos(theta/2)), from one sincos of the half angle after reduction in double
+// (FP32 sincos for FP32 results: both parts keep FP32 relative precision
+// also for tiny theta)
 template <class T>
 VP_HD cx<T> cis_neg_m1(double th) {
   const double r = th - 6.283185307179586 * rint(th * 0.15915494309189535);
-  const double h = sin(0.5 * r);
-  return mk<T>((T)(-2.0 * h * h), (T)(-sin(r)));
+  if constexpr (sizeof(T) == 4) {
+    float s, c;
+    sincosf((float)(0.5 * r), &s, &c);
+    return mk<T>(-2.0f * s * s, -2.0f * s * c);
+  } else {
+    double s, c;
+    sincos(0.5 * r, &s, &c);
+    return mk<T>(-2.0 * s * s, -2.0 * s * c);
+  }
 }
 
 // PhaseMul on a stored deviation f - B from a fixed background B whose

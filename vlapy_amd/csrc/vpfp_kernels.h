@@ -156,6 +156,13 @@ struct FieldArgs {
   double* e_out2;         // [nx] or null
   double* series;         // [2] or null: mean(E^2), mean(driver^2)
   int series_stride;      // batched scan: doubles between consecutive instances' series entries
+  // single grid (vpfp_step_n_store): workgroups xm_blk0 and up of the launch
+  // form the first xm_n x-modes of xm_f [nx][xm_nv] (FP32 if xm_f32, else
+  // FP64) into xm_out [xm_n][xm_nv] (xmodes_body); xm_tw[r] = exp(-2 pi i r / nx)
+  const void* xm_f;
+  int xm_f32, xm_n, xm_nv, xm_blk0;
+  const cx<double>* xm_tw;
+  cx<double>* xm_out;
 };
 
 // The view of instance inst of a batched field solve: every array is
@@ -416,10 +423,14 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
     }
   }
   if (chunk) {
-    // natural order -> contiguous chunk of L samples per thread
+    // natural order -> contiguous chunk of L samples per thread; after a
+    // kick buf already holds the row in natural order (the inverse
+    // transform's last pass ends by reading v from it)
+    if (!(a.flags & RK_KICK_A)) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) buf[pidx(t + i * TPT)] = v[i];
-    c.sync();
+      for (int i = 0; i < Q; ++i) buf[pidx(t + i * TPT)] = v[i];
+      c.sync();
+    }
     double d[L];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -720,16 +731,26 @@ VP_HD void drift_body(Ctx& c, unsigned char* lds, const DriftArgs<T>& a) {
   const int b0 = a.xcd_order ? xcd_block(c.bid, c.nblk) : c.bid;
   const int per = (ntiles + c.nblk - 1) / c.nblk;
   const int t0 = b0 * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
-  for (int tile = t0; tile < t1; ++tile) {
-    const int col = tile * C + cc;
-    const T* fi = a.f_in + foff + col;
-    cx<T> v[Q];
+  // the next tile's columns are loaded before this tile's transforms, so
+  // its reads are in flight during them
+  cx<T> v[Q], nxt[Q];
+  double vc_next = 0.0;
+  auto load = [&](int tile, cx<T>* dst, double& vcol) {
+    const T* fi = a.f_in + foff + tile * C + cc;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const int j = t + i * TPT;
-      v[i] = mk<T>(fi[(size_t)(2 * j) * nv], fi[(size_t)(2 * j + 1) * nv]);
+      dst[i] = mk<T>(fi[(size_t)(2 * j) * nv], fi[(size_t)(2 * j + 1) * nv]);
     }
-    const double vc = a.v[col];
+    vcol = a.v[tile * C + cc];
+  };
+  if (t0 < t1) load(t0, nxt, vc_next);
+  for (int tile = t0; tile < t1; ++tile) {
+    const int col = tile * C + cc;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) v[i] = nxt[i];
+    const double vc = vc_next;
+    if (tile + 1 < t1) load(tile + 1, nxt, vc_next);
     c.sync();  // twiddle tables staged (first tile) / buffer free (later tiles)
     spectral_apply<T, M, TPT>(c, v, buf, tw, t, PhaseMul<T>((dk * a.dt) * vc, M, t, TPT));
     T* fo = a.f_out + foff + col;
@@ -949,11 +970,17 @@ VP_HD void xmodes_body(Ctx& c, const T* f, int nx, int nv, int nm, const cx<doub
   if (idx >= nm * nv) return;
   const int m = idx / nv, col = idx % nv;
   double re = 0.0, im = 0.0;
+  // r = m x mod nx by increments: the loads of f and of the twiddles do not
+  // wait on an index division, so consecutive iterations overlap
+  int r = 0;
+#pragma unroll 16
   for (int x = 0; x < nx; ++x) {
-    const cx<double> w = tw[(int)(((long long)m * x) % nx)];
+    const cx<double> w = tw[r];
     const double fv = (double)f[(size_t)x * nv + col];
     re += fv * w.re;
     im += fv * w.im;
+    r += m;
+    if (r >= nx) r -= nx;
   }
   out[idx] = mk<double>(re, im);
 }

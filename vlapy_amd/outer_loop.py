@@ -8,9 +8,8 @@ arrays, so the reference's storage manager and diagnostics can consume it.
 With the leapfrog integrator, exponential pushers and the spectral field
 solve, a batch of steps runs as one call of the fused device loop
 vpfp_step_n_store (three kernels per step, f resident in HBM, the per-step f
-store written by the row kernel; a fourth launch per step for the k-mode
-store); any other combination runs the per-operator closures of
-vlapy_amd.core step by step.
+store or its x-modes written by the row kernel); any other combination runs
+the per-operator closures of vlapy_amd.core step by step.
 """
 
 import numpy as np
@@ -113,7 +112,14 @@ def _fused_eligible(all_params):
 
 
 class _FusedInnerLoop:
-    """Batches of leapfrog + collision + storage steps on the device."""
+    """Batches of leapfrog + collision + storage steps on the device.
+
+    One vpfp_step_n_store call per batch (three launches per step; the row
+    kernel writes the per-step f store), then one synchronisation: the E
+    history, the moments and the f store come back by asynchronous copies
+    into page-locked host arrays that this stepper owns and puts into
+    temp_storage (reused by every batch, as the reference's storage step
+    overwrites its arrays in place)."""
 
     def __init__(self, all_params, stuff):
         nu = float(all_params["nu"])
@@ -130,48 +136,91 @@ class _FusedInnerLoop:
         self.driver_function = stuff["driver_function"]
         self.rules = stuff["rules_to_store_f"]
         self.nx, self.nv = int(stuff["nx"]), int(stuff["nv"])
+        self._bufs = {}
+
+    def _driver(self, time_array):
+        """driver_function(t + dt) for every step, (n, nx): one call on a
+        column of times when the function broadcasts over it (the driver of
+        field_driver does), checked against single calls at the first and
+        last step; otherwise one call per step, as the reference."""
+        t = np.asarray(time_array, dtype=np.float64) + self.dt
+        n, nx = t.size, self.nx
+
+        def one(ti):
+            return np.broadcast_to(np.asarray(self.driver_function(ti), dtype=np.float64), (nx,))
+
+        try:
+            d = np.broadcast_to(np.asarray(self.driver_function(t[:, None]), dtype=np.float64), (n, nx))
+            for i in sorted({0, n - 1}):
+                ref = one(t[i])
+                if np.max(np.abs(d[i] - ref)) > 1e-14 * np.max(np.abs(ref)):
+                    raise ValueError
+            return d
+        except (ValueError, TypeError, IndexError):
+            return np.stack([one(ti) for ti in t])
+
+    def _buffers(self, n, f_dtype, dev):
+        """Device outputs and page-locked host arrays for batches of n steps."""
+        key = (n, f_dtype)
+        if key not in self._bufs:
+            nx, nv = self.nx, self.nv
+            space = self.rules["space"]
+            nm = 0 if space == "all" else len(space)
+            pin = dict(pin_memory=True)
+            b = {"nm": nm,
+                 "e_hist": torch.empty(n, nx, dtype=torch.float64, device=dev),
+                 "mom": torch.empty(n, NMOMENTS, nx, dtype=torch.float64, device=dev),
+                 "h_e": torch.empty(n, nx, dtype=torch.float64, **pin),
+                 "h_mom": torch.empty(NMOMENTS, n, nx, dtype=torch.float64, **pin)}
+            if nm == 0:
+                b["store"] = torch.empty(n, nx, nv, dtype=f_dtype, device=dev)
+                b["h_store"] = torch.empty(n, nx, nv, dtype=torch.float64, **pin)
+            else:
+                b["store"] = torch.empty(n, nm, nv, dtype=torch.complex128, device=dev)
+                b["h_store"] = torch.empty(n, nm, nv, dtype=torch.complex64, **pin)
+            self._bufs[key] = b
+        return self._bufs[key]
 
     def __call__(self, time_array, driver_array, temp_storage):
         dev = _dev.device()
         n = len(time_array)
-        nx = self.nx
         temp_storage["time_batch"] = time_array
         temp_storage["driver_array_batch"] = driver_array
         f = _dev.to_dev(temp_storage["f"], _dev.f_dtype(self.precision))
         e = _dev.to_dev(temp_storage["e"], torch.float64)
-        drv = np.stack([np.broadcast_to(np.asarray(self.driver_function(t + self.dt), dtype=np.float64), (nx,))
-                        for t in time_array])
-        drv_d = torch.as_tensor(drv, device=dev) if np.any(drv != 0.0) else None
-        e_hist = torch.empty(n, nx, dtype=torch.float64, device=dev)
-        mom = torch.empty(n, NMOMENTS, nx, dtype=torch.float64, device=dev)
-        space = self.rules["space"]
-        if space == "all":
-            stored = torch.empty((n, nx, self.nv), dtype=f.dtype, device=dev)
+        drv = self._driver(time_array)
+        drv_d = torch.as_tensor(np.ascontiguousarray(drv), device=dev) if np.any(drv != 0.0) else None
+        b = self._buffers(n, f.dtype, dev)
+        # the whole batch in one fused call; the row kernel writes f (or adds
+        # its rows' x-modes) after every step's collision into the store
+        ops.step_n_store(self.plan, f, e, n, b["store"], b["nm"], drv_d, b["e_hist"], b["mom"])
+        b["h_e"].copy_(b["e_hist"], non_blocking=True)
+        b["h_mom"].copy_(b["mom"].transpose(0, 1), non_blocking=True)
+        if b["nm"]:
+            b["h_store"].copy_(b["store"].to(torch.complex64), non_blocking=True)
         else:
-            nm = len(self.rules)
-            stored = torch.empty((n, nm, self.nv), dtype=torch.complex128, device=dev)
-        # the whole batch in one fused call; the row kernel writes f (or its
-        # x-modes) after every step's collision into `stored`
-        ops.step_n_store(self.plan, f, e, n, stored, 0 if space == "all" else stored.shape[1], drv_d, e_hist, mom)
-        # back to the host storage
-        eh = e_hist.cpu().numpy()
-        mo = mom.cpu().numpy()
+            b["h_store"].copy_(b["store"].double(), non_blocking=True)
+        f_host = f.double().cpu().numpy()  # synchronises the stream
+        eh, mo = b["h_e"].numpy(), b["h_mom"].numpy()
         flds, ser = temp_storage["fields"], temp_storage["series"]
-        flds["e"][:] = eh
+        flds["e"] = eh
         flds["driver"][:] = np.asarray(driver_array)
         for key, row in (("n", 0), ("j", 1), ("T", 2), ("q", 3), ("fv4", 4), ("vN", 5)):
-            flds[key][:] = mo[:, row]
-        ser["mean_n"][:] = flds["n"].mean(axis=1)
-        ser["mean_j"][:] = flds["j"].mean(axis=1)
-        ser["mean_T"][:] = flds["T"].mean(axis=1)
+            flds[key] = mo[row]
+        ser["mean_n"][:] = mo[0].mean(axis=1)
+        ser["mean_j"][:] = mo[1].mean(axis=1)
+        ser["mean_T"][:] = mo[2].mean(axis=1)
         ser["mean_e2"][:] = np.mean(eh ** 2.0, axis=1)
         ser["mean_de2"][:] = np.mean(np.asarray(driver_array) ** 2.0, axis=1)
-        ser["mean_f2"][:] = mo[:, 6].mean(axis=1)
-        ser["mean_flogf"][:] = mo[:, 7].mean(axis=1)
-        sf = stored.cpu().numpy()
-        temp_storage["stored_f"][:] = sf.astype(temp_storage["stored_f"].dtype, copy=False)
+        ser["mean_f2"][:] = mo[6].mean(axis=1)
+        ser["mean_flogf"][:] = mo[7].mean(axis=1)
+        hs = b["h_store"].numpy()
+        if hs.dtype == temp_storage["stored_f"].dtype and hs.shape == temp_storage["stored_f"].shape:
+            temp_storage["stored_f"] = hs
+        else:
+            temp_storage["stored_f"][:] = hs.astype(temp_storage["stored_f"].dtype, copy=False)
         temp_storage["e"] = eh[-1].copy()
-        temp_storage["f"] = f.double().cpu().numpy()
+        temp_storage["f"] = f_host
         post_inner_loop_update(temp_storage, np)
         return temp_storage
 

@@ -3,11 +3,13 @@ uneven (nx, nv), wavenumber, amplitude and collision frequency advanced
 together on one GPU.
 
 Instances are grouped by size class (nx, nv); each class is one batch of the
-C ABI (vpfp_batch_*), whose f lives in HBM as [n_inst][nx][nv].  A step of a
-class is three launches: the fused kick-collide-kick rows of every instance,
-the drift of every instance's columns (with per-workgroup partial densities),
-and the field solve of every instance.  Classes run concurrently on separate
-HIP streams.
+C ABI (vpfp_batch_*), whose f lives in HBM as [n_inst][nx][nv].  By default
+the classes form one group (vpfp_group_*) stepped on one stream: one launch
+of the fused kick-collide-kick rows per distinct nv, one drift (with
+per-workgroup partial densities) and one field launch per distinct nx, so
+the 12 classes of a 4096-instance draw take 10 launches per step.  With
+grouped=False each class takes its own three launches (rows, drift, field)
+on its own HIP stream.
 """
 
 import ctypes
