@@ -988,6 +988,7 @@ def dropin_section(steps=NSTEPS, steps_in_loop=1000):
     (outer_loop.get_inner_loop_stepper with backend "hip", FP64): batches of
     steps_in_loop steps, each filling the reference's temp_storage (fields,
     series, the k0/k1 f store) on the host.  ms per step, beside step_n."""
+    steps_in_loop = min(steps_in_loop, steps)
     import numpy as np
     import torch
 

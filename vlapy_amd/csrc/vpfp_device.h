@@ -200,8 +200,10 @@ VP_HD void dft(cx<T>* x) {
 // global layout [lo: 64 entries | hi: max(1, N/64) entries].
 // ---------------------------------------------------------------------------
 // For N <= kTwFull the whole table W_N^i, i < N, is staged instead: one LDS
-// read per twiddle.  (A full FP32 table for N = 2048, 16 KiB, measured no
-// faster than the two-level one at nv = 2048.)
+// read per twiddle and no complex product.  Larger full tables cost more than
+// they save: every workgroup stages its table from L2, and a full FP32 table
+// for N = 2048 (16 KiB per one-row workgroup) doubled the row kernel's L2
+// reads and measured 6 % slower at nv = 2048 than the two-level one.
 constexpr int kTwFull = 1024;
 VP_HD constexpr bool tw_full(int N, int bytes) { return bytes > 0 && N <= kTwFull; }
 VP_HD constexpr int tw_entries(int N, int bytes) {
