@@ -849,6 +849,7 @@ def main():
     ap.add_argument("--large-steps", type=int, default=200)
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-closure", action="store_true")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the reference inner-loop timing of the EPW run")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (for counter runs that should see only the timed shapes)")
     ap.add_argument("--copy-bytes", type=int, default=None,
@@ -932,7 +933,8 @@ def main():
                                "one_core_value": scan["cpu_baseline"]["one_core"]["instances_per_s"]}
     if not args.no_epw:
         out["epw"] = epw_section(args.epw_steps, args.epw_warmup, min(args.profile_steps, args.epw_steps),
-                                 not args.no_autotune, None if args.no_cpu else args.cpu_seconds)
+                                 not args.no_autotune, None if args.no_cpu else args.cpu_seconds,
+                                 dropin=not args.no_dropin)
     if not args.no_large:
         out["large_grid"] = large_section(args.large_steps, not args.no_autotune, not args.no_cpu)
     if not args.no_epw:
@@ -1038,7 +1040,7 @@ def dropin_section(steps=NSTEPS, steps_in_loop=1000):
             "step_n_store_ms_per_step": dev_wall / steps * 1e3, "vs_step_n_store": wall / dev_wall}
 
 
-def epw_section(steps, warmup, profile_steps, autotune, cpu_seconds):
+def epw_section(steps, warmup, profile_steps, autotune, cpu_seconds, dropin=True):
     """The collisional EPW (nx=64, nv=1024) in FP32, FP64 and FP32 delta-f,
     each `steps` fused steps with the storage moments."""
     import numpy as np
@@ -1067,9 +1069,10 @@ def epw_section(steps, warmup, profile_steps, autotune, cpu_seconds):
     out["fp64"]["graph_ms_per_step"] = epw_graph("fp64", steps)
     out["fp32"]["graph_ms_per_step"] = epw_graph("fp32", steps)
     # the reference's own inner-loop interface on the same workload (FP64)
-    d = dropin_section(steps)
-    d["vs_step_n"] = d["ms_per_step"] / out["fp64"]["ms_per_step"]
-    out["dropin_fp64"] = d
+    if dropin:
+        d = dropin_section(steps)
+        d["vs_step_n"] = d["ms_per_step"] / out["fp64"]["ms_per_step"]
+        out["dropin_fp64"] = d
     return out
 
 

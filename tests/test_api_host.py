@@ -157,3 +157,30 @@ def test_plan_background_validates_f_before_the_device_call():
                 torch.zeros(8, 4, dtype=torch.float64).t(), torch.zeros(4, 8, dtype=torch.float64)):
         with pytest.raises(ValueError):
             Plan.background(fake, bad, 1)
+
+
+def test_fused_inner_loop_driver_batch_equals_per_step_calls():
+    """The fused inner loop evaluates driver_function(t + dt) for a whole
+    batch at once when the function broadcasts over a column of times, and
+    falls back to one call per step otherwise; both give the per-step values
+    the reference's leapfrog step uses (vlasov_poisson.py:55)."""
+    from vlapy_amd import field_driver
+    from vlapy_amd.outer_loop import _FusedInnerLoop
+
+    g = O.Grid.make(16, 8, 2 * np.pi / 0.3, 6.0)
+    pulses = {"p": {"k0": 0.3, "w0": 1.16, "t_L": 2.0, "t_R": 20.0, "t_wL": 2.5, "t_wR": 2.5, "a0": 1e-2}}
+    t = 0.1 * np.arange(40)
+
+    class Fake:
+        dt, nx = 0.1, 16
+
+    for fn in (field_driver.get_driver_function(g.x, pulses),
+               lambda tt: O.driver_field(g.x, pulses, tt),
+               lambda tt: np.full(16, float(tt)) * 1e-3,      # does not broadcast: per-step fallback
+               lambda tt: np.zeros(16)):                       # time-independent
+        f = Fake()
+        f.driver_function = fn
+        got = _FusedInnerLoop._driver(f, t)
+        want = np.stack([np.broadcast_to(np.asarray(fn(ti + 0.1), dtype=np.float64), (16,)) for ti in t])
+        assert got.shape == (40, 16)
+        np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-300)

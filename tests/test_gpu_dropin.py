@@ -219,3 +219,42 @@ def test_inner_loop_matches_oracle(integrator, rule):
     else:
         want = np.fft.fft(ts_o["f"], axis=0)[:2]
         np.testing.assert_allclose(ts_a["stored_f"][-1], want.astype(np.complex64), rtol=1e-5, atol=1e-6)
+
+
+def test_inner_loop_two_batches_match_oracle():
+    """Two consecutive batches through the fused inner loop (its output
+    arrays are page-locked buffers reused by the next batch, as the
+    reference's storage step overwrites its arrays in place): after each
+    batch f, E, the series, the cumulative driver energy and the k0/k1 store
+    equal the oracle's inner loop run batch by batch."""
+    from vlapy_amd import initializers, outer_loop
+
+    nx, nv, k, nt = 16, 128, 0.3, 5
+    ap = initializers.make_default_params_dictionary()
+    ap.update({"nx": nx, "nv": nv, "nu": 5e-3, "xmax": 2 * np.pi / k, "tmax": 2.0, "nt": 21})
+    ap["fokker-planck"]["type"] = "dg"
+    rule = {"time": "all", "space": ["k0", "k1"]}
+
+    class D:
+        rules_to_store_f = rule
+
+    pulses = {"p": {"k0": k, "w0": 1.16, "t_L": 0.0, "t_R": 20.0, "t_wL": 2.5, "t_wR": 2.5, "a0": 1e-2}}
+    stuff = outer_loop.get_everything_ready_for_outer_loop(D, ap, pulses, 2 * nt)
+    stuff["f"] = stuff["f"] * (1 + 0.05 * np.cos(k * stuff["x"]))[:, None]
+    fused = outer_loop.get_inner_loop_stepper(ap, stuff, nt)
+    ts_a = outer_loop.get_arrays_for_inner_loop(stuff, nt, rule)
+    g = O.Grid(stuff["x"], stuff["kx"], stuff["one_over_kx"], stuff["v"], stuff["kv"], stuff["dv"])
+    ts_o = O.new_temp_storage(stuff["e"], stuff["f"], nt)
+    for b in range(2):
+        sl = slice(b * nt, (b + 1) * nt)
+        tarr, darr = stuff["t"][sl], stuff["driver"][sl]
+        ts_a = fused(tarr, darr, ts_a)
+        ts_o = O.inner_loop(g, ts_o, tarr, darr, stuff["dt"], 5e-3, "dg", "leapfrog", stuff["driver_function"])
+        print("batch", b, flush=True)
+        assert rel(ts_a["f"], ts_o["f"]) < 1e-11
+        assert rel(ts_a["fields"]["e"], ts_o["fields"]["e"]) < 1e-10
+        for key in ("mean_n", "mean_T", "mean_e2", "mean_de2", "mean_cum_de2"):
+            np.testing.assert_allclose(ts_a["series"][key], ts_o["series"][key], rtol=1e-10, err_msg=key)
+        assert ts_a["mean_cum_de2_previous"] == pytest.approx(ts_o["mean_cum_de2_previous"], rel=1e-10)
+        want = np.fft.fft(ts_o["stored_f"], axis=1)[:, :2]
+        np.testing.assert_allclose(ts_a["stored_f"], want.astype(np.complex64), rtol=1e-5, atol=1e-6)

@@ -954,12 +954,14 @@ struct TuneScratch {
   }
 };
 
-std::vector<int> row_q_candidates(int nv) {
+// values per thread that exist for this row length (delta-f rows have no
+// 16-value form: q = 16 would run the 8-value kernel again)
+std::vector<int> row_q_candidates(int nv, bool delta = false) {
   const int M = nv / 2;
   std::vector<int> q;
   if (M >= 16 && M <= 2048) q.push_back(4);
   q.push_back(8);
-  if (M >= 256 && M <= 4096) q.push_back(16);
+  if (M >= 256 && M <= 4096 && !delta) q.push_back(16);
   return q;
 }
 
@@ -1021,7 +1023,7 @@ int autotune_plan(vpfp_plan_t p, int n_steps, hipStream_t s) {
   TuneScratch w;
   if (int rc = w.init(tune_state<T>(p->nx, p->nx, p->nv, p->v0, p->dvv), 2 * (size_t)p->nx)) return rc;
   std::vector<Shape> cands;
-  for (int q : row_q_candidates(p->nv))
+  for (int q : row_q_candidates(p->nv, p->delta != 0))
     for (int C : {4, 8, 16, 32}) {
       if (C > p->nv) continue;
       for (int fuse : {0, 1, 2}) {
@@ -1057,7 +1059,7 @@ int autotune_batch(vpfp_batch_s* b, int n_steps, hipStream_t s) {
   const size_t rows = (size_t)b->n * b->nx;
   if (int rc = w.init(tune_state<T>(rows, b->nx, b->nv, b->v0, b->dvv), rows)) return rc;
   std::vector<Shape> cands;
-  for (int q : row_q_candidates(b->nv))
+  for (int q : row_q_candidates(b->nv, b->delta != 0))
     for (int C : {4, 8, 16, 32}) {
       if (C > b->nv) continue;
       for (int xo : {0, 1})
@@ -1308,7 +1310,7 @@ int autotune_group(vpfp_group_s* g, int n_steps, hipStream_t s) {
   for (auto& v : g->vg) {
     double best = 1e300;
     int bq = v.q;
-    for (int q : row_q_candidates(v.nv)) {
+    for (int q : row_q_candidates(v.nv, g->b[v.members[0]]->delta != 0)) {
       double us = 0.0;
       const int r = time([&]() { return row_multi_dispatch<T>(group_rows<T>(g, v, w.f.data(), w.e.data(), flags), s, q); },
                          &us);

@@ -120,7 +120,7 @@ __global__ __launch_bounds__((vp::DriftCfg<T, M, Q, C>::wg)) void k_drift(vp::Dr
 
 template <class T, int M, int C>
 int launch_drift_c(const vp::DriftArgs<T>& a, hipStream_t s, int gx, int gy) {
-  constexpr int Q = M <= 64 ? (M < 4 ? M : 4) : 8;
+  constexpr int Q = vp::drift_q(M);
   using Cfg = vp::DriftCfg<T, M, Q, C>;
   auto kern = k_drift<T, M, Q, C>;
   if (Cfg::wg > 1024 || Cfg::lds > kMaxLds)
@@ -168,7 +168,7 @@ __global__ __launch_bounds__((vp::DriftCfg<T, M, Q, C>::wg)) void k_drift_multi(
 
 template <class T, int M, int C>
 int launch_drift_multi_c(vp::DriftMulti<T> m, hipStream_t s) {
-  constexpr int Q = M <= 64 ? (M < 4 ? M : 4) : 8;
+  constexpr int Q = vp::drift_q(M);
   using Cfg = vp::DriftCfg<T, M, Q, C>;
   auto kern = k_drift_multi<T, M, Q, C>;
   if (Cfg::wg > 1024 || Cfg::lds > kMaxLds)

@@ -689,6 +689,10 @@ struct DriftArgs {
 // halves of a 128-byte line read by neighbouring tiles meet in one L2.
 VP_HD int xcd_block(int b, int n) { return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3); }
 
+// values per thread of the x-advection transforms (8 for M = 32 and 64,
+// two Stockham passes instead of three, measured no faster on the scan)
+VP_HD constexpr int drift_q(int M) { return M <= 64 ? (M < 4 ? M : 4) : 8; }
+
 template <class T, int M, int Q, int C>
 struct DriftCfg {
   static constexpr int TPT = M / Q;

@@ -189,7 +189,7 @@ class _FusedInnerLoop:
         f = _dev.to_dev(temp_storage["f"], _dev.f_dtype(self.precision))
         e = _dev.to_dev(temp_storage["e"], torch.float64)
         drv = self._driver(time_array)
-        drv_d = torch.as_tensor(np.ascontiguousarray(drv), device=dev) if np.any(drv != 0.0) else None
+        drv_d = torch.as_tensor(np.array(drv), device=dev) if np.any(drv != 0.0) else None  # writable copy
         b = self._buffers(n, f.dtype, dev)
         # the whole batch in one fused call; the row kernel writes f (or adds
         # its rows' x-modes) after every step's collision into the store
