@@ -406,9 +406,14 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
     else
       spectral_apply<T, M, TPT>(c, v, buf, tw, t, PhaseMul<T>(a.dk * shift, M, t, TPT));
   };
-  if (a.flags & RK_KICK_A) kick(a.dt_a * e);
-
   const bool chunk = (a.flags & (RK_COLLIDE | RK_MOMENTS)) != 0;
+  // two half kicks by the same field with nothing between them (no
+  // collision, moments or f store: the collisionless fused step) are one
+  // kick by the sum of the shifts: the phase factors multiply exactly, so
+  // one transform pair replaces two
+  const bool merged = (a.flags & RK_KICK_A) && (a.flags & RK_KICK_B) && !chunk && !a.f_copy;
+  if (a.flags & RK_KICK_A) kick((merged ? a.dt_a + a.dt_b : a.dt_a) * e);
+
   if (a.f_copy && !chunk && valid) {
     // copy only: natural order, one complex pair per value (coalesced)
     cx<T>* dst = reinterpret_cast<cx<T>*>(a.f_copy + (size_t)row * a.nv);
@@ -506,7 +511,7 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
     c.sync();
   }
 
-  if (a.flags & RK_KICK_B) kick(a.dt_b * e);
+  if ((a.flags & RK_KICK_B) && !merged) kick(a.dt_b * e);
 
   if (valid && a.f_out) {
     cx<T>* dst = reinterpret_cast<cx<T>*>(a.f_out + (size_t)row * a.nv);
