@@ -985,11 +985,12 @@ def epw_graph(precision, steps, chunk=100):
     return a.elapsed_time(b) / (reps * chunk)
 
 
-def dropin_section(steps=NSTEPS, steps_in_loop=1000):
+def dropin_section(steps=NSTEPS, steps_in_loop=1000, scheme="leapfrog"):
     """The EPW workload through the reference's inner-loop interface
     (outer_loop.get_inner_loop_stepper with backend "hip", FP64): batches of
     steps_in_loop steps, each filling the reference's temp_storage (fields,
-    series, the k0/k1 f store) on the host.  ms per step, beside step_n."""
+    series, the k0/k1 f store) on the host.  ms per step, beside the device
+    loop alone (step_n_store, or step_n_scheme for "pefrl" / "h-sixth")."""
     steps_in_loop = min(steps_in_loop, steps)
     import numpy as np
     import torch
@@ -1000,6 +1001,7 @@ def dropin_section(steps=NSTEPS, steps_in_loop=1000):
     ap.update({"nx": NX, "nv": NV, "nu": NU, "xmin": 0.0, "xmax": 2 * np.pi / K0, "vmax": VMAX,
                "tmax": DT * steps, "nt": steps + 1})
     ap["fokker-planck"]["type"] = "dg"
+    ap["vlasov-poisson"]["time"] = scheme
 
     class D:
         rules_to_store_f = {"space": ["k0", "k1"], "time": "all"}
@@ -1028,16 +1030,24 @@ def dropin_section(steps=NSTEPS, steps_in_loop=1000):
     eh = torch.empty(steps_in_loop, NX, dtype=torch.float64, device=dev)
     mo = torch.empty(steps_in_loop, NMOMENTS, NX, dtype=torch.float64, device=dev)
     st = torch.empty(steps_in_loop, 2, NV, dtype=torch.complex128, device=dev)
-    ops.step_n_store(stepper.plan, f, e, steps_in_loop, st, 2, None, eh, mo)
+
+    def device_batch():
+        if scheme == "leapfrog":
+            ops.step_n_store(stepper.plan, f, e, steps_in_loop, st, 2, None, eh, mo)
+        else:
+            ops.step_n_scheme(stepper.plan, scheme, f, e, steps_in_loop, None, eh, mo, None, st, 2)
+
+    device_batch()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     for b in range(steps // steps_in_loop):
-        ops.step_n_store(stepper.plan, f, e, steps_in_loop, st, 2, None, eh, mo)
+        device_batch()
     torch.cuda.synchronize()
     dev_wall = time.perf_counter() - t1
-    return {"workload": WORKLOAD + "; reference inner-loop interface, %d-step batches, k0/k1 f store" % steps_in_loop,
+    return {"workload": WORKLOAD + "; %s; reference inner-loop interface, %d-step batches, k0/k1 f store"
+                                   % (scheme, steps_in_loop),
             "ms_per_step": wall / steps * 1e3, "steps": steps,
-            "step_n_store_ms_per_step": dev_wall / steps * 1e3, "vs_step_n_store": wall / dev_wall}
+            "device_loop_ms_per_step": dev_wall / steps * 1e3, "vs_device_loop": wall / dev_wall}
 
 
 def epw_section(steps, warmup, profile_steps, autotune, cpu_seconds, dropin=True):
@@ -1073,6 +1083,9 @@ def epw_section(steps, warmup, profile_steps, autotune, cpu_seconds, dropin=True
         d = dropin_section(steps)
         d["vs_step_n"] = d["ms_per_step"] / out["fp64"]["ms_per_step"]
         out["dropin_fp64"] = d
+        # the 4th-order PEFRL and 6th-order splittings through the same interface
+        for scheme in ("pefrl", "h-sixth"):
+            out["dropin_%s_fp64" % scheme] = dropin_section(min(steps, 1000), scheme=scheme)
     return out
 
 

@@ -161,6 +161,27 @@ int vpfp_step_n(vpfp_plan_t plan, void* f, double* e, int n_steps, const double*
 int vpfp_step_n_store(vpfp_plan_t plan, void* f, double* e, int n_steps, const double* driver, double* e_hist,
                       double* moments, double* series, void* f_store, int n_modes, void* stream);
 
+/* n_steps of a higher-order splitting, each followed by the collision step,
+ * as one device loop: the 4th-order PEFRL (VPFP_SCHEME_PEFRL, replaces
+ * full_pefrl_ps_step, vlasov_poisson.py:63-154: five drifts each followed by a
+ * field solve, four kicks) or the 6th-order splitting (VPFP_SCHEME_SIXTH,
+ * replaces sixth_order_step, vlasov_poisson.py:157-232: kicks around five
+ * drifts with their field solves), with the collision step of step.py:321-322
+ * after each.  The last kick of a step, the collision, the moments, the f store
+ * and the next step's first kick share one row-kernel launch.
+ * driver [n_steps][VPFP_SCHEME_FIELDS][nx]: the driver at the step's field
+ * solves in order (the reference's driver_function(t + ...) at each; NULL =
+ * zero).  e [nx] is the field the 6th-order scheme's first kick uses and
+ * becomes the last E.  e_hist [n_steps][nx] (E after each step), moments
+ * [n_steps][VPFP_NMOMENTS][nx], series [n_steps][2] (mean E^2 and mean square
+ * of the step's last driver) and f_store/n_modes as vpfp_step_n_store (the
+ * x-modes by one launch per step) are optional (NULL / 0). */
+#define VPFP_SCHEME_PEFRL 1
+#define VPFP_SCHEME_SIXTH 2
+#define VPFP_SCHEME_FIELDS 5
+int vpfp_step_n_scheme(vpfp_plan_t plan, int scheme, void* f, double* e, int n_steps, const double* driver,
+                       double* e_hist, double* moments, double* series, void* f_store, int n_modes, void* stream);
+
 /* Trapz moments of f per x after the step: moments [VPFP_NMOMENTS][nx] =
  * n, j, T, q, fv4, vN (integrals of f v^p, p = 0..5) then f^2 and f log f.
  * Replaces update_fields / the f-integrals of update_series, step.py:153-224. */

@@ -173,6 +173,7 @@ def test_fused_inner_loop_driver_batch_equals_per_step_calls():
 
     class Fake:
         dt, nx = 0.1, 16
+        _driver_at = _FusedInnerLoop._driver_at
 
     for fn in (field_driver.get_driver_function(g.x, pulses),
                lambda tt: O.driver_field(g.x, pulses, tt),
@@ -184,3 +185,34 @@ def test_fused_inner_loop_driver_batch_equals_per_step_calls():
         want = np.stack([np.broadcast_to(np.asarray(fn(ti + 0.1), dtype=np.float64), (16,)) for ti in t])
         assert got.shape == (40, 16)
         np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-300)
+
+
+def test_scheme_field_times_follow_the_core_integrators():
+    """The fused PEFRL / 6th-order batches evaluate the driver at the times
+    the per-operator integrators use: PEFRL's running sum of the drift
+    lengths, the 6th-order scheme's t + a_k dt."""
+    from vlapy_amd.core import vlasov_poisson as VP
+    from vlapy_amd.outer_loop import _scheme_field_times
+
+    seen = []
+
+    def drift(f, dt=None, **kw):
+        return f
+
+    def kick(f, e=None, dt=None, **kw):
+        return f
+
+    def field(driver_field=None, f=None):
+        return driver_field
+
+    t = np.array([0.0, 0.3, 1.7])
+    for name in ("pefrl", "h-sixth"):
+        seen.clear()
+        step = VP.get_time_integrator(name, drift, kick, field,
+                                      {"dt": 0.1, "driver_function": lambda tt: seen.append(tt) or 0.0})
+        want = []
+        for ti in t:
+            seen.clear()
+            step(0.0, np.zeros(4), ti)
+            want.append(list(seen))
+        np.testing.assert_array_equal(_scheme_field_times(name, t, 0.1), np.array(want))

@@ -186,10 +186,12 @@ def test_device_resident_mode_matches_numpy_mode(z):
 
 @pytest.mark.parametrize("integrator,rule", [("leapfrog", {"space": "all"}),
                                               ("leapfrog", {"time": "first-last", "space": ["k0", "k1"]}),
-                                              ("pefrl", {"space": "all"})])
+                                              ("pefrl", {"space": "all"}),
+                                              ("h-sixth", {"time": "first-last", "space": ["k0", "k1"]})])
 def test_inner_loop_matches_oracle(integrator, rule):
-    """outer_loop with the HIP backend against the oracle's inner loop:
-    leapfrog runs the fused device loop, PEFRL the per-operator closures."""
+    """outer_loop with the HIP backend against the oracle's inner loop: each
+    batch is one fused device loop (vpfp_step_n_store for leapfrog,
+    vpfp_step_n_scheme for PEFRL and the 6th-order splitting)."""
     from vlapy_amd import field_driver, initializers, outer_loop
 
     nx, nv, k, nt = 16, 128, 0.3, 6

@@ -256,3 +256,70 @@ def test_unsupported_sizes_raise(dev):
         ops.edfdv(p, f, f, torch.zeros(6, dtype=torch.float64, device=dev), 0.1)
     with pytest.raises(ValueError):
         ops.edfdv(p, f, f, torch.zeros(5, dtype=torch.float64, device=dev), 0.1)
+
+
+@pytest.mark.parametrize("scheme,nu,delta,n_modes", [("pefrl", 1e-2, False, 0), ("pefrl", 0.0, False, 2),
+                                                      ("h-sixth", 1e-2, False, 0), ("h-sixth", 0.0, False, 0),
+                                                      ("h-sixth", 1e-2, True, 2)])
+def test_step_n_scheme_matches_oracle(dev, scheme, nu, delta, n_modes):
+    """vpfp_step_n_scheme (PEFRL / 6th-order splitting, each step followed by
+    the Dougherty collision) against the oracle's pefrl_step / hsixth_step and
+    collide over 6 steps with a driver and a non-zero initial field: the E
+    history, the moments after every step, the f store and f, FP64.  The
+    fused kicks (last kick, collision, next step's first kick in one launch;
+    one kick by the summed shift without a collision) change only rounding."""
+    import torch
+    from vlapy_amd import ops
+    from vlapy_amd.outer_loop import _scheme_field_times
+
+    nx, nv, k, dt, n = 32, 256, 0.35, 0.1, 6
+    print("step_n_scheme", scheme, "nu", nu, "delta", delta, "modes", n_modes, flush=True)
+    g = O.Grid.make(nx, nv, 2 * np.pi / k, 6.0)
+    f0 = O.perturbed_maxwellian(g.x, g.v, k, 0.2)
+    e0 = O._field(g, 0.0, f0)
+    pulses = {"p": {"k0": k, "w0": 1.2, "t_L": 0.0, "t_R": 20.0, "t_wL": 2.5, "t_wR": 2.5, "a0": 1e-2}}
+    drv_fn = lambda t: O.driver_field(g.x, pulses, t)
+    step = O.INTEGRATORS[scheme]
+    f, e = f0.copy(), e0.copy()
+    eh, mo, st = [], [], []
+    for i in range(n):
+        e, f = step(g, e, f, i * dt, dt, drv_fn)
+        f = O.collide(f, g.v, nu, dt, g.dv, "dg")
+        eh.append(e.copy())
+        m = O.step_moments(f, g.v, g.dv)
+        mo.append(np.stack([m[key] for key in ("n", "j", "T", "q", "fv4", "vN")]))
+        st.append(np.fft.fft(f, axis=0)[:n_modes] if n_modes else f.copy())
+    times = _scheme_field_times(scheme, dt * np.arange(n), dt)
+    drv = np.stack([[drv_fn(tt) for tt in row] for row in times])
+    p = make_plan(g, nx, nv, nu=nu, dt=dt, op="dg")
+    m_bg = np.exp(-g.v ** 2 / 2.0) / np.sqrt(2 * np.pi)
+    if delta:
+        p.set_delta(True)
+    fd = T(f0 - m_bg[None, :] if delta else f0, dev)
+    ed = T(e0, dev)
+    ehd = torch.zeros(n, nx, dtype=torch.float64, device=dev)
+    mod = torch.zeros(n, 8, nx, dtype=torch.float64, device=dev)
+    std = (torch.zeros(n, n_modes, nv, dtype=torch.complex128, device=dev) if n_modes
+           else torch.zeros(n, nx, nv, dtype=torch.float64, device=dev))
+    ops.step_n_scheme(p, scheme, fd, ed, n, T(drv, dev), ehd, mod, None, std, n_modes)
+    got_f = fd.cpu().numpy() + (m_bg[None, :] if delta else 0.0)
+    tol = 1e-11 if delta else 1e-12
+    assert rel(got_f, f) < tol
+    assert rel(ehd.cpu().numpy(), np.array(eh)) < 1e-10
+    assert rel(ed.cpu().numpy(), eh[-1]) < 1e-10
+    assert rel(mod[:, :6].cpu().numpy(), np.array(mo)) < 1e-11
+    assert rel(std.cpu().numpy(), np.array(st)) < tol
+
+
+def test_step_n_scheme_refuses_leapfrog_and_bad_shapes(dev):
+    import torch
+    from vlapy_amd import ops
+
+    g = O.Grid.make(16, 64, 2 * np.pi / 0.3, 6.0)
+    p = make_plan(g, 16, 64, nu=1e-2, dt=0.1, op="dg")
+    f = T(O.perturbed_maxwellian(g.x, g.v, 0.3, 0.1), dev)
+    e = torch.zeros(16, dtype=torch.float64, device=dev)
+    with pytest.raises(NotImplementedError):
+        ops.step_n_scheme(p, "leapfrog", f, e, 2)
+    with pytest.raises(ValueError):
+        ops.step_n_scheme(p, "pefrl", f, e, 2, torch.zeros(2, 4, 16, dtype=torch.float64, device=dev))

@@ -90,6 +90,7 @@ SIGNATURES = {
     "vpfp_collide": [_c_p, _c_p, _c_p, _c_p],
     "vpfp_step_n": [_c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p],
     "vpfp_step_n_store": [_c_p, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i, _c_p],
+    "vpfp_step_n_scheme": [_c_p, _c_i, _c_p, _c_p, _c_i, _c_p, _c_p, _c_p, _c_p, _c_p, _c_i, _c_p],
     "vpfp_moments": [_c_p, _c_p, _c_p, _c_p],
     "vpfp_x_modes": [_c_p, _c_p, _c_i, _c_p, _c_p],
     "vpfp_batch_create": [ctypes.POINTER(BatchDesc), ctypes.POINTER(_c_p)],

@@ -749,6 +749,108 @@ int do_step_n(vpfp_plan_t p, void* f, double* e, int n, const double* driver, do
   return prof_collect(p->prof);
 }
 
+constexpr int kSchemeFields = VPFP_SCHEME_FIELDS;  // field solves per PEFRL or 6th-order step
+
+// One step of a higher-order splitting as a list of stages: a kick by c dt
+// with the latest field, or a drift by c dt followed by a field solve.
+struct Stage {
+  bool kick;
+  double c;
+};
+
+// PEFRL (vlasov_poisson.py:99-154, Omelyan, Mryglod & Folk 2002) and the
+// 6th-order splitting of Casas et al. 2017 (vlasov_poisson.py:191-232):
+// published method constants
+std::vector<Stage> scheme_stages(int scheme, double dt) {
+  if (scheme == VPFP_SCHEME_PEFRL) {
+    const double xsi = 0.1786178958448091, lam = -0.2123418310626054, chi = -0.6626458266981849e-1;
+    const double kw0 = 0.5 * (1.0 - 2.0 * lam);
+    return {{false, xsi}, {true, kw0}, {false, chi}, {true, lam}, {false, 1.0 - 2.0 * (chi + xsi)},
+            {true, lam},  {false, chi}, {true, kw0}, {false, xsi}};
+  }
+  const double a1 = 0.168735950563437422448196, a2 = 0.377851589220928303880766,
+               a3 = -0.093175079568731452657924, b1 = 0.049086460976116245491441,
+               b2 = 0.264177609888976700200146, b3 = 0.186735929134907054308413,
+               c1 = -0.000069728715055305084099, c2 = -0.000625704827430047189169,
+               c3 = -0.002213085124045325561636, d2 = -2.916600457689847816445691e-6,
+               d3 = 3.048480261700038788680723e-5, e3 = 4.985549387875068121593988e-7;
+  const double k1 = b1 + 2.0 * c1 * std::pow(dt, 2.0);
+  const double k2 = b2 + 2.0 * c2 * std::pow(dt, 2.0) + 4.0 * d2 * std::pow(dt, 4.0);
+  const double k3 = b3 + 2.0 * c3 * std::pow(dt, 2.0) + 4.0 * d3 * std::pow(dt, 4.0) - 8.0 * e3 * std::pow(dt, 6.0);
+  return {{true, k1},  {false, a1}, {true, k2}, {false, a2}, {true, k3},  {false, a3},
+          {true, k3},  {false, a2}, {true, k2}, {false, a1}, {true, k1}};
+}
+
+// n steps of a scheme, each followed by the collision step, on the device:
+// drifts with their field solve, kicks through the row kernel; the last
+// kick of a step, the collision, the moments, the f store and the next
+// step's first kick share one row launch.  driver [n][kSchemeFields][nx].
+template <class T>
+int do_step_scheme(vpfp_plan_t p, int scheme, void* f, double* e, int n, const double* driver, double* e_hist,
+                   double* moments, double* series, hipStream_t s, void* f_store, int n_modes) {
+  const std::vector<Stage> st = scheme_stages(scheme, p->dt);
+  const int ns = (int)st.size();
+  const bool coll = p->nu > 0.0;
+  const bool first_kick = st[0].kick, last_kick = st[ns - 1].kick;
+  auto* modes = reinterpret_cast<cx<double>*>(f_store);
+  const double* e_cur = e;
+  long long field_no = 0;
+  if (n > 0 && first_kick) {
+    auto ra = row_args<T>(p, f, f);
+    ra.e = e;
+    ra.dt_a = st[0].c * p->dt;
+    ra.flags = vp::RK_KICK_A;
+    PROF(KK_ROW, row_dispatch<T>(ra, s, p->row_q));
+  }
+  for (int i = 0; i < n; ++i) {
+    int j = 0;  // field solve of this step
+    for (int k = 0; k < ns; ++k) {
+      const Stage& g = st[k];
+      if (g.kick) {
+        if (k == 0 || k == ns - 1) continue;  // fused into the row launch that ends a step
+        auto rk = row_args<T>(p, f, f);
+        rk.e = e_cur;
+        rk.dt_a = g.c * p->dt;
+        rk.flags = vp::RK_KICK_A;
+        PROF(KK_ROW, row_dispatch<T>(rk, s, p->row_q));
+        continue;
+      }
+      PROF(KK_DRIFT, (do_vdfdx<T>(p, f, f, g.c * p->dt, p->dens, s)));
+      const bool last = j == kSchemeFields - 1;
+      double* eo = (last && e_hist) ? e_hist + (size_t)i * p->nx : p->ering + (size_t)(field_no & 1) * p->nx;
+      const double* drv = driver ? driver + ((size_t)i * kSchemeFields + j) * p->nx : nullptr;
+      double* ser = (last && series) ? series + 2 * (size_t)i : nullptr;
+      auto fa = field_args(p, p->dens, p->ntiles, p->dv, drv, eo, nullptr, ser, true);
+      PROF(KK_FIELD, field_dispatch(fa, true, s));
+      e_cur = eo;
+      ++j;
+      ++field_no;
+    }
+    auto rb = row_args<T>(p, f, f);
+    rb.e = e_cur;
+    rb.dt_a = last_kick ? st[ns - 1].c * p->dt : 0.0;
+    rb.dt_b = st[0].c * p->dt;
+    rb.moments = moments ? moments + (size_t)i * vp::kMoments * p->nx : nullptr;
+    rb.flags = (last_kick ? vp::RK_KICK_A : 0) | (coll ? vp::RK_COLLIDE : 0) | (moments ? vp::RK_MOMENTS : 0) |
+               (first_kick && i + 1 < n ? vp::RK_KICK_B : 0);
+    if (f_store)
+      rb.f_copy = n_modes > 0 ? reinterpret_cast<T*>(p->fcopy)
+                              : reinterpret_cast<T*>(f_store) + (size_t)i * p->nx * p->nv;
+    if (rb.flags || rb.f_copy) PROF(KK_ROW, row_dispatch<T>(rb, s, p->row_q));
+    if (f_store && n_modes > 0) {
+      const int total = n_modes * p->nv;
+      if (int rc = prof_mark(p->prof, KK_OTHER, s)) return rc;
+      hipLaunchKernelGGL(k_xmodes<T>, dim3((total + 255) / 256), dim3(256), 0, s,
+                         reinterpret_cast<const T*>(p->fcopy), p->nx, p->nv, n_modes, p->tw_xd,
+                         modes + (size_t)i * n_modes * p->nv);
+      if (int rc = launched()) return rc;
+      if (int rc = prof_mark(p->prof, KK_OTHER, s)) return rc;
+    }
+  }
+  if (n > 0) VPFP_HIP(hipMemcpyAsync(e, e_cur, sizeof(double) * p->nx, hipMemcpyDeviceToDevice, s));
+  return prof_collect(p->prof);
+}
+
 }  // namespace
 
 // ====================================================================== batched scan
@@ -2138,6 +2240,28 @@ int vpfp_group_kernel_times(vpfp_group_t g, double* ms, long long* counts) {
     counts[k] = g->prof.n[k];
   }
   return VPFP_OK;
+}
+
+int vpfp_step_n_scheme(vpfp_plan_t p, int scheme, void* f, double* e, int n, const double* driver, double* e_hist,
+                       double* moments, double* series, void* f_store, int n_modes, void* stream) {
+  if (int rc = check_plan(p)) return rc;
+  if (scheme != VPFP_SCHEME_PEFRL && scheme != VPFP_SCHEME_SIXTH)
+    return fail(VPFP_EBADARG, "scheme must be VPFP_SCHEME_PEFRL or VPFP_SCHEME_SIXTH");
+  if (!f || !e) return fail(VPFP_EBADARG, "null array");
+  if (n < 0) return fail(VPFP_EBADARG, "negative step count");
+  if (n_modes < 0 || n_modes > p->nx) return fail(VPFP_EBADARG, "n_modes must be in [0, nx]");
+  if (n_modes > 0 && !f_store) return fail(VPFP_EBADARG, "n_modes > 0 needs f_store");
+  if (int rc = need_spectral_v(p)) return rc;
+  if (int rc = need_spectral_x(p)) return rc;
+  if (p->nx > 2048) return fail(VPFP_EUNSUPPORTED, "nx too large for the fused step");
+  if (n_modes > 0 && !p->fcopy) {
+    if (int rc = dalloc(p, &p->fcopy, (size_t)p->nx * p->nv * (p->prec == VPFP_F64 ? 8 : 4))) return rc;
+  }
+  return p->prec == VPFP_F64
+             ? do_step_scheme<double>(p, scheme, f, e, n, driver, e_hist, moments, series, p->streams.use(stream),
+                                      f_store, n_modes)
+             : do_step_scheme<float>(p, scheme, f, e, n, driver, e_hist, moments, series, p->streams.use(stream),
+                                     f_store, n_modes);
 }
 
 int vpfp_step_n_store(vpfp_plan_t p, void* f, double* e, int n, const double* driver, double* e_hist, double* moments,

@@ -124,6 +124,36 @@ def step_n_store(plan, f, e, n_steps, f_store, n_modes=0, driver=None, e_hist=No
     return f, e
 
 
+SCHEMES = {"pefrl": 1, "h-sixth": 2}
+SCHEME_FIELDS = 5
+
+
+def step_n_scheme(plan, scheme, f, e, n_steps, driver=None, e_hist=None, moments=None, series=None, f_store=None,
+                  n_modes=0):
+    """n steps of the PEFRL ("pefrl") or 6th-order ("h-sixth") splitting, each
+    followed by the collision step (vpfp_step_n_scheme): driver (n, 5, nx)
+    float64 at the step's five field solves; e_hist, moments, series as
+    step_n; f_store as step_n_store."""
+    if scheme not in SCHEMES:
+        raise NotImplementedError("df/dt : <" + str(scheme) + "> has not yet been implemented")
+    nx, nv = plan.nx, plan.nv
+    n = int(n_steps)
+    nm = int(n_modes)
+    drv = _check(driver, (n, SCHEME_FIELDS, nx), torch.float64, "driver") if driver is not None else None
+    eh = _check(e_hist, (n, nx), torch.float64, "e_hist") if e_hist is not None else None
+    mo = _check(moments, (n, _lib.NMOMENTS, nx), torch.float64, "moments") if moments is not None else None
+    se = _check(series, (n, 2), torch.float64, "series") if series is not None else None
+    st = None
+    if f_store is not None:
+        st = (_check(f_store, (n, nx, nv), f_dtype(plan), "f_store") if nm == 0
+              else _check(f_store, (n, nm, nv), torch.complex128, "f_store"))
+    elif nm:
+        raise ValueError("n_modes > 0 needs f_store")
+    _lib.check(_lib.library().vpfp_step_n_scheme(plan.handle, SCHEMES[scheme], _f(plan, f), _x(plan, e, "e"), n, drv,
+                                                 eh, mo, se, st, nm, _stream()))
+    return f, e
+
+
 def moments(plan, f, out):
     """Trapz moments [8][nx]: n, j, T, q, fv4, vN, f^2, f log f."""
     _lib.check(_lib.library().vpfp_moments(plan.handle, _f(plan, f),

@@ -5,11 +5,14 @@ accepts only "numpy", outer_loop.py:49-56).  The inner loop keeps the
 reference's temp_storage schema (outer_loop.py:149-215) with host numpy
 arrays, so the reference's storage manager and diagnostics can consume it.
 
-With the leapfrog integrator, exponential pushers and the spectral field
-solve, a batch of steps runs as one call of the fused device loop
-vpfp_step_n_store (three kernels per step, f resident in HBM, the per-step f
-store or its x-modes written by the row kernel); any other combination runs
-the per-operator closures of vlapy_amd.core step by step.
+With exponential pushers and the spectral field solve, a batch of steps runs
+as one call of a fused device loop: vpfp_step_n_store for leapfrog (three
+kernels per step, f resident in HBM, the per-step f store or its x-modes
+written by the row kernel), vpfp_step_n_scheme for PEFRL and the 6th-order
+splitting (their drifts, field solves and kicks as device launches, the
+collision, moments and f store fused into the row launch that ends a step).
+Any other combination runs the per-operator closures of vlapy_amd.core step
+by step.
 """
 
 import numpy as np
@@ -17,7 +20,7 @@ import torch
 
 from . import field_driver, initializers, ops
 from ._lib import NMOMENTS, Plan
-from .core import _dev, step
+from .core import _dev, step, vlasov_poisson
 
 BACKENDS = ("hip",)
 
@@ -107,15 +110,37 @@ def post_inner_loop_update(temp_storage, this_np=np):
 
 def _fused_eligible(all_params):
     vp = all_params["vlasov-poisson"]
-    return (vp["time"] == "leapfrog" and vp["vdfdx"] == "exponential" and vp["edfdv"] == "exponential"
-            and vp["poisson"] == "spectral")
+    return (vp["time"] in ("leapfrog",) + tuple(ops.SCHEMES) and vp["vdfdx"] == "exponential"
+            and vp["edfdv"] == "exponential" and vp["poisson"] == "spectral")
+
+
+def _scheme_field_times(scheme, time_array, dt):
+    """(n, 5) times of the five field solves of each step, as the core
+    integrators form them: PEFRL the running sum t + dt1 + dt2 + ...
+    (vlasov_poisson.py:115-146), the 6th-order scheme t + a_k dt
+    (vlasov_poisson.py:206-226)."""
+    t = np.asarray(time_array, dtype=np.float64)
+    out = np.empty((t.size, ops.SCHEME_FIELDS))
+    if scheme == "pefrl":
+        w = (vlasov_poisson._XSI, vlasov_poisson._CHI, 1.0 - 2.0 * (vlasov_poisson._CHI + vlasov_poisson._XSI),
+             vlasov_poisson._CHI, vlasov_poisson._XSI)
+        tc = t.copy()
+        for j in range(ops.SCHEME_FIELDS):
+            tc = tc + w[j] * dt
+            out[:, j] = tc
+    else:
+        a1, a2, a3 = vlasov_poisson._H6[:3]
+        for j, a in enumerate((a1, a2, a3, a2, a1)):
+            out[:, j] = t + a * dt
+    return out
 
 
 class _FusedInnerLoop:
-    """Batches of leapfrog + collision + storage steps on the device.
+    """Batches of (leapfrog, PEFRL or 6th-order) + collision + storage steps
+    on the device.
 
-    One vpfp_step_n_store call per batch (three launches per step; the row
-    kernel writes the per-step f store), then one synchronisation: the E
+    One vpfp_step_n_store (leapfrog: three launches per step) or
+    vpfp_step_n_scheme call per batch, then one synchronisation: the E
     history, the moments and the f store come back by asynchronous copies
     into page-locked host arrays that this stepper owns and puts into
     temp_storage (reused by every batch, as the reference's storage step
@@ -136,14 +161,19 @@ class _FusedInnerLoop:
         self.driver_function = stuff["driver_function"]
         self.rules = stuff["rules_to_store_f"]
         self.nx, self.nv = int(stuff["nx"]), int(stuff["nv"])
+        self.scheme = all_params["vlasov-poisson"]["time"]
         self._bufs = {}
 
     def _driver(self, time_array):
-        """driver_function(t + dt) for every step, (n, nx): one call on a
+        """driver_function(t + dt) for every step, (n, nx)."""
+        return self._driver_at(np.asarray(time_array, dtype=np.float64) + self.dt)
+
+    def _driver_at(self, t):
+        """driver_function at each time of t, (t.size, nx): one call on a
         column of times when the function broadcasts over it (the driver of
         field_driver does), checked against single calls at the first and
-        last step; otherwise one call per step, as the reference."""
-        t = np.asarray(time_array, dtype=np.float64) + self.dt
+        last time; otherwise one call per time, as the reference."""
+        t = np.asarray(t, dtype=np.float64).reshape(-1)
         n, nx = t.size, self.nx
 
         def one(ti):
@@ -188,12 +218,20 @@ class _FusedInnerLoop:
         temp_storage["driver_array_batch"] = driver_array
         f = _dev.to_dev(temp_storage["f"], _dev.f_dtype(self.precision))
         e = _dev.to_dev(temp_storage["e"], torch.float64)
-        drv = self._driver(time_array)
+        if self.scheme == "leapfrog":
+            drv = self._driver(time_array)
+        else:
+            drv = self._driver_at(_scheme_field_times(self.scheme, time_array, self.dt)).reshape(
+                n, ops.SCHEME_FIELDS, self.nx)
         drv_d = torch.as_tensor(np.array(drv), device=dev) if np.any(drv != 0.0) else None  # writable copy
         b = self._buffers(n, f.dtype, dev)
-        # the whole batch in one fused call; the row kernel writes f (or adds
-        # its rows' x-modes) after every step's collision into the store
-        ops.step_n_store(self.plan, f, e, n, b["store"], b["nm"], drv_d, b["e_hist"], b["mom"])
+        # the whole batch in one fused call; the row kernel writes f (or its
+        # copy for the x-modes) after every step's collision into the store
+        if self.scheme == "leapfrog":
+            ops.step_n_store(self.plan, f, e, n, b["store"], b["nm"], drv_d, b["e_hist"], b["mom"])
+        else:
+            ops.step_n_scheme(self.plan, self.scheme, f, e, n, drv_d, b["e_hist"], b["mom"], None, b["store"],
+                              b["nm"])
         b["h_e"].copy_(b["e_hist"], non_blocking=True)
         b["h_mom"].copy_(b["mom"].transpose(0, 1), non_blocking=True)
         if b["nm"]:
