@@ -33,6 +33,7 @@ import sys
 import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+CLOSURE_NPZ = os.path.join(HERE, "vlapy_amd", "data", "closure_c5.npz")  # the trained closure
 METRIC = "VPFP cell-updates/s + % HBM roofline; scan runs/s at ≤1% damping-rate error"
 HBM_PEAK_GBS = 8000.0
 
@@ -796,7 +797,7 @@ def closure_section():
 
     from vlapy_amd import closure as CL
 
-    path = os.path.join(HERE, "vlapy_amd", "data", "closure_c5.npz")
+    path = CLOSURE_NPZ
     if not os.path.exists(path):
         return None
     net = CL.Closure.load(path, max_rows=64)
@@ -829,7 +830,37 @@ def closure_section():
         out[name] = r
     out["speedup_vs_fine_fp64"] = {k: out[k]["instances_per_s"] / out["fine_fp64"]["instances_per_s"]
                                    for k in ("coarse_fp32", "coarse_fp32_closure", "coarse_fp32_delta_f")}
+    out["network"] = closure_network_rate(path)
     return out
+
+
+def closure_network_rate(path, rows=262144, reps=5):
+    """The closure's inference kernel alone: `rows` coarse rows (the
+    4096-instance scan at nx = 64) through Closure.apply, HIP events over
+    `reps` calls; TFLOP/s of its convolutions (2 * 5 * (64 + 64 * 64 * 2 + 64)
+    * 256 flops per row) against the 2.5 PFLOP/s dense BF16 peak."""
+    import numpy as np
+    import torch
+
+    from vlapy_amd import closure as CL
+
+    net = CL.Closure.load(path, max_rows=rows)
+    v = np.linspace(-6.4 + 0.025, 6.4 - 0.025, 256)
+    rng = np.random.default_rng(7)
+    f = (np.exp(-v ** 2 / 2)[None, :] / np.sqrt(2 * np.pi)) * (1 + 1e-2 * rng.standard_normal((rows, 256)))
+    x = torch.as_tensor(f.astype(np.float32), device="cuda:0")
+    net.apply(x)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        net.apply(x)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    flops = 2.0 * 5 * (64 + 64 * 64 * 2 + 64) * 256 * rows
+    tf = flops / (ms / 1e3) / 1e12
+    return {"rows": rows, "ms_per_call": ms, "tflops": tf, "bf16_peak_tflops": 2500.0, "frac": tf / 2500.0}
 
 
 def main():
@@ -850,6 +881,8 @@ def main():
     ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-closure", action="store_true")
     ap.add_argument("--no-dropin", action="store_true", help="skip the reference inner-loop timing of the EPW run")
+    ap.add_argument("--closure-network", action="store_true",
+                    help="only time the closure's inference kernel (262144 rows) and print its line")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (for counter runs that should see only the timed shapes)")
     ap.add_argument("--copy-bytes", type=int, default=None,
@@ -871,6 +904,9 @@ def main():
         return 0
     if args.scan_cpu_child is not None:
         scan_cpu_child(args.scan_cpu_child)
+        return 0
+    if args.closure_network:
+        print(json.dumps({"closure_network": closure_network_rate(CLOSURE_NPZ)}))
         return 0
     if args.copy_bytes is not None:
         import torch

@@ -119,6 +119,17 @@ __device__ __forceinline__ void stage_p(bf16* P, const bf16* src) {
 
 // ----------------------------------------------------------------------------
 // forward: one workgroup (256 threads, thread = position) per row, rows strided
+//
+// All four layers run on MFMA over one position-major LDS tile P:
+//  * layer 1 (1 -> 64, K = 5): row p+2 of P keeps, in its pad columns 64..71,
+//    the stencil record x[p-2..p+2] (BF16, zero outside the row, three zero
+//    slots); wave w multiplies it by W1[16w + i][k] (A fragment with K < 5);
+//  * layers 2 and 3 (64 -> 64): mconv, then in place over P once every wave
+//    holds its sums (the records in columns 64..71 are not touched);
+//  * layer 4 (64 -> 1, K = 320): W4 in every row of the A operand, so each
+//    lane's sums are y at its own position; wave w takes positions
+//    64w..64w+63 and hands y to the position's thread through LDS.
+// The activations get tanh, rounded to BF16, as they leave the accumulators.
 // ----------------------------------------------------------------------------
 struct FwdArgs {
   const float* f;       // [rows][256]
@@ -145,7 +156,10 @@ struct FwdArgs {
   float* loss_rows;     // [rows]: sum_v (y - target)^2
 };
 
-constexpr int kFwdLds = 2 * kRowsP * kP * 2 + kRowsP * 4 + 64 + 2 * kK * 4;
+// the position-major tile (with the layer-1 stencil records in its pad
+// columns) + reduction slots + BF16 W4 + layer-1 A rows + the layer-4
+// outputs: 40192 bytes, so three workgroups share a CU's 160 KiB
+constexpr int kFwdLds = kRowsP * kP * 2 + 64 + kK * 2 + kCH * 16 + kNV * 4;
 
 // sum over the 256 threads of a workgroup (all threads get it); red: 4 floats
 __device__ __forceinline__ float block_sum(float v, float* red) {
@@ -157,22 +171,67 @@ __device__ __forceinline__ float block_sum(float v, float* red) {
   return (red[0] + red[1]) + (red[2] + red[3]);
 }
 
-__global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
-  extern __shared__ __align__(16) unsigned char lds[];
-  bf16* P0 = reinterpret_cast<bf16*>(lds);
-  bf16* P1 = P0 + kRowsP * kP;
-  float* xs = reinterpret_cast<float*>(P1 + kRowsP * kP);
-  float* red = xs + kRowsP;
-  float* w1r = red + 16;   // BF16-rounded W1 [64][5]
-  float* w4r = w1r + kK;   // BF16-rounded W4 [5][64]
-  const int t = threadIdx.x;
-  zero_pad_p(P0);
-  zero_pad_p(P1);
-  if (t < 4) xs[t < 2 ? t : kNV + t] = 0.f;
-  for (int i = t; i < kK; i += 256) {
-    w1r[i] = bfr(a.p[oW1 + i]);
-    w4r[i] = bfr(a.p[oW4 + i]);
+// tanh for activations that are rounded to BF16 next, accurate to ~3e-7
+// relative (a one-ulp BF16 flip moves the outputs it feeds by ~1e-3 of max |y|,
+// so this error sets how often GPU and oracle activations round apart):
+// sign(x) (1 - e) / (1 + e) with e = exp(-2|x|) from the hardware 2^x and
+// reciprocal, and for |x| < 0.2, where 1 - e cancels, the odd Taylor series
+// x - x^3/3 + 2x^5/15 - 17x^7/315 (truncation <= 62x^9/2835 ~ 6e-8 relative).
+// Both sides are computed and selected, without a branch.
+__device__ __forceinline__ float tanh_act(float x) {
+  const float ax = fabsf(x);
+  const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * ax);  // -2 log2(e) |x|
+  const float r = (1.f - e) * __builtin_amdgcn_rcpf(1.f + e);
+  const float x2 = x * x;
+  const float p = ax * fmaf(x2, fmaf(x2, fmaf(x2, -0.0539682540f, 0.133333333f), -0.333333333f), 1.f);
+  return copysignf(ax < 0.2f ? p : r, x);
+}
+
+// tanh(acc + b) of accumulator tile nt (channels co0..co0+3 of lane (m, g),
+// position 16 nt + m) into P at pr = P + (m + 2) kP + co0 and, when training,
+// into the stored activations at gr = gout + (row kCH + co0) kNV + m
+template <int NT>
+__device__ __forceinline__ void act_tile(const floatx4& acc, const float (&b)[4], bf16* pr, bf16* gr) {
+  bf16x4 o;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) o[j] = (bf16)tanh_act(acc[j] + b[j]);
+  *reinterpret_cast<bf16x4*>(pr + 16 * NT * kP) = o;
+  if (gr) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) gr[j * kNV + 16 * NT] = o[j];
   }
+}
+
+// act_tile over the 16 tiles of a 64 -> 64 layer
+template <int NT = 0>
+__device__ __forceinline__ void act_tiles(const floatx4* acc, const float (&b)[4], bf16* pr, bf16* gr) {
+  if constexpr (NT < 16) {
+    act_tile<NT>(acc[NT], b, pr, gr);
+    act_tiles<NT + 1>(acc, b, pr, gr);
+  }
+}
+
+// three waves per SIMD (<= 168 registers) to match the three workgroups the
+// LDS allows; left to itself the compiler takes 260 registers (one wave per
+// SIMD), which measured 2.4x slower at 262144 rows
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_fwd(FwdArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  bf16* P = reinterpret_cast<bf16*>(lds);
+  float* red = reinterpret_cast<float*>(P + kRowsP * kP);
+  bf16* w4b = reinterpret_cast<bf16*>(red + 16);  // BF16 W4, K order k*64 + ci
+  bf16* w1b = w4b + kK;                            // [64][8]: W1[co][k], k < 5, then zeros
+  float* ys = reinterpret_cast<float*>(w1b + 8 * kCH);  // layer-4 output by position
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, m = lane & 15, g = lane >> 4;
+  bf16x8 z8;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) z8[j] = (bf16)0.f;
+  zero_pad_p(P);
+  *reinterpret_cast<bf16x8*>(P + (t + 2) * kP + kCH) = z8;  // stencil records start at zero
+  for (int i = t; i < kK; i += 256) w4b[i] = (bf16)a.p[oW4 + i];
+  for (int i = t; i < 8 * kCH; i += 256) w1b[i] = (bf16)((i & 7) < kKS ? a.p[oW1 + (i >> 3) * kKS + (i & 7)] : 0.f);
+  const int co0 = 16 * wave + 4 * g;
+  bf16* const pr = P + (m + 2) * kP + co0;  // this lane's activation slots (tile 0)
+  __syncthreads();  // records zeroed before any thread writes into them
   for (int row = blockIdx.x; row < a.rows; row += gridDim.x) {
     const size_t ro = (size_t)row * kNV;
     const float fv = a.f[ro + t];
@@ -182,77 +241,79 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
       d = fv - a.bg[t];
       srow = sqrtf(block_sum(d * d, red) * (1.f / kNV) + 1e-30f);
     }
-    const float x = bfr((a.in_scale / srow) * d);
-    xs[t + 2] = x;
-    if (a.xin) a.xin[ro + t] = (bf16)x;
-    __syncthreads();
-    // layer 1: 1 -> 64, tanh
-    {
-      float xv[kKS];
+    const bf16 xb = (bf16)((a.in_scale / srow) * d);
+    if (a.xin) a.xin[ro + t] = xb;
+    // x[t] is entry k of the record of position t + 2 - k
 #pragma unroll
-      for (int k = 0; k < kKS; ++k) xv[k] = xs[t + k];
-#pragma unroll
-      for (int c0 = 0; c0 < kCH; c0 += 8) {
-        bf16x8 o;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int c = c0 + j;
-          float s = 0.f;
-#pragma unroll
-          for (int k = 0; k < kKS; ++k) s = fmaf(w1r[c * kKS + k], xv[k], s);
-          o[j] = (bf16)tanhf(s + a.p[oB1 + c]);
-          if (a.a1) a.a1[((size_t)row * kCH + c) * kNV + t] = o[j];
-        }
-        *reinterpret_cast<bf16x8*>(P0 + (t + 2) * kP + c0) = o;
-      }
+    for (int k = 0; k < kKS; ++k) {
+      const int p = t + 2 - k;
+      if (p >= 0 && p < kNV) P[(p + 2) * kP + kCH + k] = xb;
     }
     __syncthreads();
-    // layers 2 and 3: 64 -> 64 on MFMA, tanh
-    auto layer = [&](const bf16* in, bf16* out, const bf16x8 (&W)[10], const float* b, bf16* gout) {
-      mconv(in, W, [&](floatx4* acc, int co0, int m) {
-        const float b0 = b[co0], b1 = b[co0 + 1], b2 = b[co0 + 2], b3 = b[co0 + 3];
+    // layer 1: 1 -> 64, tanh (writes columns 0..63 only, while the records are read)
+    {
+      // A[i][k] = W1[16 wave + i][k] for k < 5, zero beyond (lanes g > 0 read
+      // the zero padding row of P)
+      const bf16x8 A1 = *reinterpret_cast<const bf16x8*>(g == 0 ? w1b + 8 * (16 * wave + m) : P);
+      const float b[4] = {a.p[oB1 + co0], a.p[oB1 + co0 + 1], a.p[oB1 + co0 + 2], a.p[oB1 + co0 + 3]};
+      bf16* gr = a.a1 ? a.a1 + ((size_t)row * kCH + co0) * kNV + m : nullptr;
+      const bf16* rec = P + (m + 2) * kP + kCH;
+      floatx4 acc[16];
 #pragma unroll
-        for (int nt = 0; nt < 16; ++nt) {
-          const int pos = 16 * nt + m;
-          bf16x4 o;
-          o[0] = (bf16)tanhf(acc[nt][0] + b0);
-          o[1] = (bf16)tanhf(acc[nt][1] + b1);
-          o[2] = (bf16)tanhf(acc[nt][2] + b2);
-          o[3] = (bf16)tanhf(acc[nt][3] + b3);
-          *reinterpret_cast<bf16x4*>(out + (pos + 2) * kP + co0) = o;
-          if (gout) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) gout[((size_t)row * kCH + co0 + j) * kNV + pos] = o[j];
-          }
-        }
-      });
-    };
-    // weight fragments are reloaded per row (L2-resident): keeping both
-    // layers' fragments in registers measured slower (register pressure)
+      for (int nt = 0; nt < 16; ++nt)
+        acc[nt] = mfma(A1, *reinterpret_cast<const bf16x8*>(rec + 16 * nt * kP), floatx4{0.f, 0.f, 0.f, 0.f});
+      act_tiles(acc, b, pr, gr);
+    }
+    __syncthreads();
+    // layers 2 and 3: 64 -> 64, tanh, in place: every wave has its sums in
+    // registers before any wave overwrites the tile.  The weight fragments are
+    // reloaded per row (L2-resident): keeping both layers' in registers
+    // measured slower (register pressure)
     {
       bf16x8 A[10];
       load_a(a.w2, A);
-      layer(P0, P1, A, a.p + oB2, a.a2);
+      mconv(P, A, [&](floatx4* acc, int c0, int mm) {
+        const float b[4] = {a.p[oB2 + c0], a.p[oB2 + c0 + 1], a.p[oB2 + c0 + 2], a.p[oB2 + c0 + 3]};
+        bf16* gr = a.a2 ? a.a2 + ((size_t)row * kCH + c0) * kNV + mm : nullptr;
+        __syncthreads();
+        act_tiles(acc, b, pr, gr);
+      });
     }
     __syncthreads();
     {
       bf16x8 A[10];
       load_a(a.w3, A);
-      layer(P1, P0, A, a.p + oB3, a.a3);
+      mconv(P, A, [&](floatx4* acc, int c0, int mm) {
+        const float b[4] = {a.p[oB3 + c0], a.p[oB3 + c0 + 1], a.p[oB3 + c0 + 2], a.p[oB3 + c0 + 3]};
+        bf16* gr = a.a3 ? a.a3 + ((size_t)row * kCH + c0) * kNV + mm : nullptr;
+        __syncthreads();
+        act_tiles(acc, b, pr, gr);
+      });
     }
     __syncthreads();
-    // layer 4: 64 -> 1, linear
-    float y = a.p[oB4];
+    // layer 4: 64 -> 1, linear; every row of A is W4, so lane (m, g) sums
+    // y at position 16 nt + m in all four accumulator slots
+    {
+      floatx4 acc[4];
 #pragma unroll
-    for (int k = 0; k < kKS; ++k) {
-      const bf16* pr = P0 + (t + k) * kP;
+      for (int q = 0; q < 4; ++q) acc[q] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int c0 = 0; c0 < kCH; c0 += 8) {
-        const bf16x8 v = *reinterpret_cast<const bf16x8*>(pr + c0);
+      for (int c = 0; c < 10; ++c) {
+        const int tap = c >> 1, ci0 = (c & 1) * 32 + 8 * g;
+        const bf16x8 a4 = *reinterpret_cast<const bf16x8*>(w4b + 32 * c + 8 * g);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) y = fmaf(w4r[k * kCH + c0 + j], (float)v[j], y);
+        for (int q = 0; q < 4; ++q) {
+          const bf16x8 b = *reinterpret_cast<const bf16x8*>(P + (16 * (4 * wave + q) + m + tap) * kP + ci0);
+          acc[q] = mfma(a4, b, acc[q]);
+        }
+      }
+      if (g == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) ys[16 * (4 * wave + q) + m] = acc[q][0];
       }
     }
+    __syncthreads();
+    float y = a.p[oB4] + ys[t];
     if (a.cons) {
       // remove bg (c0 + c1 v + c2 v^2): trapz moments 1, v, v^2 of y vanish
       const float vv = a.cons[t], w = a.cons[kNV + t], b = a.cons[2 * kNV + t];
@@ -276,7 +337,6 @@ __global__ __launch_bounds__(256) void k_fwd(FwdArgs a) {
     __syncthreads();
   }
 }
-
 // ----------------------------------------------------------------------------
 // backward through a 64 -> 64 layer (input gradient):
 //   dzp[ci][p] = bf16( (sum_{k', co} Wt[ci][k'*64 + co] dz[co][p + k' - 2]) (1 - ap[ci][p]^2) )
@@ -625,7 +685,9 @@ int set_lds_all() {
 
 hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
-int fwd_grid(int rows) { return rows < 4096 ? rows : 4096; }
+// rows are strided over the workgroups: 3072 = 4 x the 3 x 256 that are
+// resident at once (measured 5 % faster than 4096, which leaves a partial wave)
+int fwd_grid(int rows) { return rows < 3072 ? rows : 3072; }
 
 int check_net(vpfc_net_t n) { return n ? VPFP_OK : fail(VPFP_EBADARG, "null network"); }
 
