@@ -7,7 +7,10 @@
 //     [rows][64][256] BF16, channel-major (v contiguous)
 //   LDS "position-major" tile of one row: [260][kP] BF16, row p+2 holds the
 //     64 channels at position p (rows 0, 1, 258, 259 are the zero padding),
-//     kP = 72 keeps every 8-channel group 16-byte aligned
+//     kP = 80 keeps every 8-channel group 16-byte aligned and the MFMA
+//     operand reads (16 positions x 16 bytes per lane group) free of LDS
+//     bank conflicts: with a 144-byte pitch (kP = 72) they measured 2-way
+//     conflicted on gfx950, with 160 bytes conflict-free
 //   LDS "channel-major" tile: [64][kQ] BF16, element p+2 of channel c at
 //     c*kQ + p + 2 (zero padding at 0, 1, 258, 259)
 //
@@ -46,7 +49,7 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int kNV = VPFC_NV, kCH = VPFC_CHANNELS, kKS = VPFC_KERNEL, kK = kCH * kKS;
-constexpr int kP = 72;          // position-major pitch (BF16)
+constexpr int kP = 80;          // position-major pitch (BF16)
 constexpr int kQ = 272;         // channel-major pitch (BF16): 16-byte rows, reads up to p + 2 + 15 < 272
 constexpr int kQs = 266;        // channel-major pitch for 2-byte column reads (133 dwords: odd)
 constexpr int kRowsP = kNV + 4;
@@ -158,7 +161,7 @@ struct FwdArgs {
 
 // the position-major tile (with the layer-1 stencil records in its pad
 // columns) + reduction slots + BF16 W4 + layer-1 A rows + the layer-4
-// outputs: 40192 bytes, so three workgroups share a CU's 160 KiB
+// outputs: 44352 bytes, so three workgroups share a CU's 160 KiB
 constexpr int kFwdLds = kRowsP * kP * 2 + 64 + kK * 2 + kCH * 16 + kNV * 4;
 
 // sum over the 256 threads of a workgroup (all threads get it); red: 4 floats
