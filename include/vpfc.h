@@ -10,8 +10,11 @@
  *     1 -> 64 (tanh) -> 64 (tanh) -> 64 (tanh) -> 1 (linear).
  * Arithmetic: every weight, activation and gradient that enters a product is
  * rounded to BF16 (round to nearest even); products accumulate in FP32; the
- * 64 -> 64 layers run on MFMA (v_mfma_f32_16x16x32_bf16); post-tanh
- * activations are stored in BF16 and tanh' = 1 - a^2 uses the stored value.
+ * forward pass runs all four layers on MFMA (v_mfma_f32_16x16x32_bf16), the
+ * backward pass the input and weight gradients of the two 64 -> 64 layers;
+ * tanh is evaluated to within 4e-7 relative
+ * before its BF16 rounding; post-tanh activations are stored in BF16 and
+ * tanh' = 1 - a^2 uses the stored value.
  *
  * Flat FP32 parameter vector (VPFC_NPARAMS = 41793 floats):
  *   W1 [64][5]   (c, k)                 offset 0
