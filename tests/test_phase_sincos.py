@@ -68,3 +68,50 @@ def test_sincos_half_fp32_error():
     # sin keeps relative precision for small arguments (the seeds of exp(-i th) - 1)
     small = np.abs(yd) < 1e-3
     assert np.all(np.abs(s[small] - np.sin(yd[small])) <= 1.0 * np.abs(yd[small]) * 2.0 ** -23)
+
+
+def _literals64():
+    src = open(SRC).read()
+    body = src[src.index("VP_HD void sincos_half(double y, double& s, double& c) {"):]
+    body = body[:body.index("\n}\n")]
+    num = r"(-?[0-9]+\.[0-9]*(?:e[-+][0-9]+)?)"
+    p0 = float(re.search(r"double p = " + num + ";", body).group(1))
+    q0 = float(re.search(r"double q = " + num + ";", body).group(1))
+    ps = [p0] + [float(v) for v in re.findall(r"p = fma\(y2, p, " + num + r"\);", body)]
+    pc = [q0] + [float(v) for v in re.findall(r"q = fma\(y2, q, " + num + r"\);", body)]
+    assert "s = fma(y * y2, p, y);" in body and "c = fma(y2, q, 1.0);" in body
+    return ps, pc
+
+
+def test_sincos_half_fp64_literals_and_error():
+    ps, pc = _literals64()
+    assert len(ps) == 11 and len(pc) == 12
+    for k, v in enumerate(ps[::-1], start=1):
+        assert abs(v - (-1) ** k / math.factorial(2 * k + 1)) <= 4e-16 * abs(v)
+    for k, v in enumerate(pc[::-1], start=1):
+        assert abs(v - (-1) ** k / math.factorial(2 * k)) <= 4e-16 * abs(v)
+    # float64 Horner (separate rounding of product and sum: at most the kernel's fma error)
+    y = np.linspace(-math.pi / 2, math.pi / 2, 20001)
+    y2 = y * y
+    p = np.full_like(y, ps[0])
+    for cf in ps[1:]:
+        p = p * y2 + cf
+    s = y + (y * y2) * p
+    q = np.full_like(y, pc[0])
+    for cf in pc[1:]:
+        q = q * y2 + cf
+    c = 1.0 + y2 * q
+    # reference: the float64 library (correctly rounded to within an ulp)
+    assert np.abs(s - np.sin(y)).max() <= 4 * 2.0 ** -52 and np.abs(c - np.cos(y)).max() <= 4 * 2.0 ** -52
+
+
+def test_reduce_2pi_constants():
+    src = open(SRC).read()
+    body = src[src.index("VP_HD double reduce_2pi(double th) {"):]
+    body = body[:body.index("\n}\n")]
+    c2, c1 = [float(v) for v in re.findall(r"fma\(-n, ([0-9.e-]+),", body)]
+    from decimal import Decimal, getcontext
+    getcontext().prec = 50
+    two_pi = Decimal("6.2831853071795864769252867665590057683943387987502")
+    assert c1 == float(two_pi)                                   # C1: 2 pi rounded to double
+    assert abs(Decimal(c1) + Decimal(c2) - two_pi) < Decimal("1e-31")  # C1 + C2 = 2 pi to ~1e-31

@@ -111,11 +111,6 @@ VP_HD double rcpp(double x) {
 // ---------------------------------------------------------------------------
 // exp(-i theta) in the working precision
 // ---------------------------------------------------------------------------
-VP_HD void cis_neg(double th, cx<double>& out) {
-  double s, c;
-  sincos(th, &s, &c);
-  out = mk<double>(c, -s);
-}
 // sin and cos of |y| <= pi/2 in FP32 from their Taylor series, Horner in y^2
 // with fma: sin to y^13, cos to y^14 (the first terms left out are below
 // 1e-9 at pi/2); absolute error ~1 ulp of 1, and sin keeps its relative
@@ -144,6 +139,52 @@ VP_HD void cis_neg(double th, cx<float>& out) {
   float s, c;
   sincos_half((float)(0.5 * r), s, c);
   out = mk<float>(fmaf(-2.f * s, s, 1.f), -2.f * s * c);
+}
+
+// theta - 2 pi n with n = rint(theta / 2 pi), two-term Cody-Waite with fma
+// (2 pi = C1 + C2, C1 its double rounding): |r| <= pi to a few ulp for the
+// arguments of the kicks and drifts
+VP_HD double reduce_2pi(double th) {
+  const double n = rint(th * 0.15915494309189535);
+  return fma(-n, 2.4492935982947064e-16, fma(-n, 6.283185307179586, th));
+}
+
+// sin and cos of |y| <= pi/2 in FP64 from their Taylor series, Horner in y^2
+// with fma: sin to y^23, cos to y^24 (the first terms left out are below
+// 1e-20 at pi/2); ~1 ulp, and sin keeps its relative precision for small y
+VP_HD void sincos_half(double y, double& s, double& c) {
+  const double y2 = y * y;
+  double p = -3.86817017063068413e-23;
+  p = fma(y2, p, 1.95729410633912626e-20);
+  p = fma(y2, p, -8.22063524662432950e-18);
+  p = fma(y2, p, 2.81145725434552060e-15);
+  p = fma(y2, p, -7.64716373181981641e-13);
+  p = fma(y2, p, 1.60590438368216133e-10);
+  p = fma(y2, p, -2.50521083854417202e-08);
+  p = fma(y2, p, 2.75573192239858925e-06);
+  p = fma(y2, p, -1.98412698412698413e-04);
+  p = fma(y2, p, 8.33333333333333322e-03);
+  p = fma(y2, p, -1.66666666666666657e-01);
+  s = fma(y * y2, p, y);
+  double q = 1.61173757109611839e-24;
+  q = fma(y2, q, -8.89679139245057408e-22);
+  q = fma(y2, q, 4.11031762331216484e-19);
+  q = fma(y2, q, -1.56192069685862253e-16);
+  q = fma(y2, q, 4.77947733238738525e-14);
+  q = fma(y2, q, -1.14707455977297245e-11);
+  q = fma(y2, q, 2.08767569878681002e-09);
+  q = fma(y2, q, -2.75573192239858883e-07);
+  q = fma(y2, q, 2.48015873015873016e-05);
+  q = fma(y2, q, -1.38888888888888894e-03);
+  q = fma(y2, q, 4.16666666666666644e-02);
+  q = fma(y2, q, -0.5);
+  c = fma(y2, q, 1.0);
+}
+
+VP_HD void cis_neg(double th, cx<double>& out) {
+  double s, c;
+  sincos_half(0.5 * reduce_2pi(th), s, c);
+  out = mk<double>(fma(-2.0 * s, s, 1.0), -2.0 * s * c);
 }
 
 // ---------------------------------------------------------------------------
@@ -364,7 +405,7 @@ VP_HD cx<T> cis_neg_m1(double th) {
     return mk<T>(-2.0f * s * s, -2.0f * s * c);
   } else {
     double s, c;
-    sincos(0.5 * r, &s, &c);
+    sincos_half(0.5 * reduce_2pi(th), s, c);
     return mk<T>(-2.0 * s * s, -2.0 * s * c);
   }
 }
