@@ -483,7 +483,7 @@ VP_HD void row_body(Ctx& c, unsigned char* lds, const RowArgs<T>& a) {
         s[5] += p;
         s[6] += wf * d[i];
         // f log f: in the storage precision (FP32 log on the FP32 path)
-        s[7] += wf * (sizeof(T) == 8 ? log(d[i]) : (double)logf((float)d[i]));
+        s[7] += wf * (sizeof(T) == 8 ? log_f64(d[i]) : (double)logf((float)d[i]));
       }
       row_sum<TPT, kMoments>(c, s, rs);
       if (valid && t == 0) {
