@@ -48,7 +48,9 @@ extern "C" {
 typedef struct vpfc_net_s* vpfc_net_t;
 
 /* A network with workspace for training batches of up to max_rows rows
- * (inference takes any number of rows). */
+ * (inference takes any number of rows).  The compute calls take a stream
+ * each and the network does not track them, so destroy and the setters
+ * below wait for the whole device (setup-time calls, not for a step loop). */
 int vpfc_create(int max_rows, vpfc_net_t* out);
 int vpfc_destroy(vpfc_net_t net);
 
