@@ -9,7 +9,7 @@ f0 = (1 + amp cos kx) exp(-v^2/2)/sqrt(2 pi), e0 = 0, dt = 0.1, 2000 leapfrog
 + collision steps, the <E^2>(t) history of every instance recorded.  The
 timed configuration stores f - M in FP32 (delta-f storage) with FP64
 moments, Poisson and tridiagonal; the same scan in FP64 is timed beside it
-and is the judge: `damping_rate` gives the fraction of fitted instances
+and is the reference: `damping_rate` gives the fraction of fitted instances
 whose damping rate agrees with the FP64 scan's within 1 %.  Plain FP32
 storage (f itself) is timed as well.
 
@@ -529,11 +529,11 @@ def host_cores():
     return min(aff, 16), aff
 
 
-def scan_section(timed_steps, warmup, with_cpu, judge=True, profile_steps=SCAN_PROFILE_STEPS):
+def scan_section(timed_steps, warmup, with_cpu, check=True, profile_steps=SCAN_PROFILE_STEPS):
     """The headline: `timed_steps` steps of the FP32 delta-f scan from the
     initial state (value = completed runs of SCAN_STEPS steps per second,
     SCAN_N * timed_steps / SCAN_STEPS / wall), then the full-length runs that
-    judge it: FP32 delta-f (the same run when timed_steps == SCAN_STEPS),
+    check it: FP32 delta-f (the same run when timed_steps == SCAN_STEPS),
     FP64 and plain FP32, SCAN_STEPS steps each."""
     import numpy as np
 
@@ -542,7 +542,7 @@ def scan_section(timed_steps, warmup, with_cpu, judge=True, profile_steps=SCAN_P
     steps = SCAN_STEPS
     head = scan_run("fp32", timed_steps, delta=True, warmup=warmup, profile_steps=max(1, profile_steps),
                     graph=(timed_steps == SCAN_STEPS))
-    if not judge:
+    if not check:
         # profiling runs: the timed steps and the kernel roofline only
         return {"headline": {"timed_steps": timed_steps, "seconds": head["wall"],
                              "runs_per_s": SCAN_N * timed_steps / SCAN_STEPS / head["wall"],
@@ -874,8 +874,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=1000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-judge", action="store_true",
-                    help="skip the full-length FP64 / FP32 scans that judge the damping rates (profiling runs)")
+    ap.add_argument("--no-check", "--no-judge", dest="no_check", action="store_true",
+                    help="skip the full-length FP64 / FP32 scans that check the damping rates (profiling runs)")
     ap.add_argument("--no-epw", action="store_true")
     ap.add_argument("--large-steps", type=int, default=200)
     ap.add_argument("--no-large", action="store_true")
@@ -929,7 +929,7 @@ def main():
     import numpy as np
 
     SCAN_TUNING["in"], SCAN_TUNING["out"] = args.scan_tuning_in, args.scan_tuning_out
-    scan = scan_section(args.steps, args.warmup, not args.no_cpu, judge=not args.no_judge,
+    scan = scan_section(args.steps, args.warmup, not args.no_cpu, check=not args.no_check,
                         profile_steps=args.scan_profile_steps)
     head = scan["headline"]
     dr = scan["fp32_delta_f"]["damping_rate"]
@@ -960,7 +960,7 @@ def main():
         "cpu_baseline": None,
         "scan": scan,
     }
-    if not args.no_judge:
+    if not args.no_check:
         scan["drift_mfma"] = drift_mfma_section(scan["fp32_delta_f"]["tuning_raw"])
     if "cpu_baseline" in scan:
         a = scan["cpu_baseline"]["all_cores"]
